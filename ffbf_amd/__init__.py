@@ -1,0 +1,412 @@
+"""MI355X feed-forward Bloom filter (FFBF) matcher -- Python side of the C ABI.
+
+The work happens in ``libffbf_amd.so`` (HIP kernels for gfx950 plus the C++
+context behind ``include/ffbf.h``); this module only binds it with ctypes and
+offers a small object interface that follows the reference's flow
+(efficient/ffbf):
+
+=====================  ==============================================  ==================
+this module            reference                                       file:line
+=====================  ==============================================  ==================
+``Matcher.build``      ``setup_bloom_search`` / ``Filter::operator<<``  greptest.cu:455-474, ffbf.cpp:268-283
+``Matcher.scan``       ``executeGrepOverlap`` / ``Filter::processFile`` greptest.cu:505-576, ffbf.cpp:310-392
+``Matcher.pattern_pass`` ``executePatternFiltering`` / ``filterPhrases`` greptest.cu:578-596, ffbf.cpp:394-485
+``Matcher.verify``     ``grep -Ff`` on the filtered outputs             src/rabin-karp/README:49-52
+``Matcher.filter``     ``bvDump``                                       greptest.cu:341-351
+=====================  ==============================================  ==================
+
+There is no CPU fallback: if the shared library is missing, importing this
+package raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffbf_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
+        "or make -C ffbf_amd/csrc)"
+    )
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+OK = 0
+EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY = -1, -2, -3, -4, -5
+FILTER_FORWARD, FILTER_REVERSE = 0, 1
+MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS = -1, 0, 1, 2
+ALPHA_ASCII, ALPHA_DNA_GC40 = 0, 1
+
+
+class FFBFError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"ffbf status {status}: {message}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("window", ctypes.c_uint32),
+        ("k", ctypes.c_uint32),
+        ("log2_bits", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("sbox", ctypes.c_uint32 * 256),
+    ]
+
+
+class Timings(ctypes.Structure):
+    _fields_ = [
+        ("build_ms", ctypes.c_float),
+        ("scan_ms", ctypes.c_float),
+        ("scan_kernel_ms", ctypes.c_float),
+        ("pattern_pass_ms", ctypes.c_float),
+        ("verify_ms", ctypes.c_float),
+        ("scan_mode", ctypes.c_uint32),
+        ("fold_log2_bits", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+MATCH_DTYPE = np.dtype([("pattern_id", "<u4"), ("reserved", "<u4"), ("offset", "<u8")])
+
+_c = ctypes
+_P = _c.c_void_p
+_u32, _u64, _i32 = _c.c_uint32, _c.c_uint64, _c.c_int
+_pu64 = _c.POINTER(_c.c_uint64)
+
+# name: (restype, argtypes) -- every symbol include/ffbf.h and include/ffbf_workload.h declare
+ABI = {
+    "ffbf_last_error": (_c.c_char_p, []),
+    "ffbf_version": (_c.c_char_p, []),
+    "ffbf_create": (_i32, [_c.POINTER(Params), _i32, _c.POINTER(_P)]),
+    "ffbf_destroy": (None, [_P]),
+    "ffbf_set_patterns": (_i32, [_P, _P, _P, _u64]),
+    "ffbf_set_patterns_lines": (_i32, [_P, _c.c_char_p, _u64]),
+    "ffbf_build": (_i32, [_P]),
+    "ffbf_scan": (_i32, [_P, _P, _u64]),
+    "ffbf_scan_device": (_i32, [_P, _P, _u64]),
+    "ffbf_run_device": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_pattern_pass": (_i32, [_P, _pu64]),
+    "ffbf_verify": (_i32, [_P, _pu64]),
+    "ffbf_get_matches": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_match_text": (_i32, [_P, _P, _u64, _c.POINTER(_P), _pu64]),
+    "ffbf_free_matches": (None, [_P]),
+    "ffbf_get_filter": (_i32, [_P, _i32, _P, _u64]),
+    "ffbf_get_candidates": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_get_survivors": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_filter_save": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_filter_load": (_i32, [_P, _P, _u64]),
+    "ffbf_set_scan_mode": (_i32, [_P, _i32, _u32]),
+    "ffbf_get_timings": (_i32, [_P, _c.POINTER(Timings)]),
+    "ffbf_synchronize": (_i32, [_P]),
+    "ffbf_get_stream": (_P, [_P]),
+    "ffbf_device_alloc": (_i32, [_i32, _u64, _c.POINTER(_P)]),
+    "ffbf_device_free": (_i32, [_P]),
+    "ffbf_device_upload": (_i32, [_P, _P, _u64]),
+    "ffbf_device_download": (_i32, [_P, _P, _u64]),
+    "ffbf_device_count": (_i32, [_c.POINTER(_i32)]),
+    # workload generators (host only)
+    "ffbf_gen_bytes": (_i32, [_u32, _u64, _u64, _u64, _P, _i32]),
+    "ffbf_gen_pattern_offsets": (_i32, [_u64, _u64, _u32, _u32, _P]),
+    "ffbf_gen_pattern_bytes": (_i32, [_u32, _u64, _u64, _P, _i32]),
+    "ffbf_gen_plants": (_i32, [_u64, _u64, _P, _u64, _P, _P, _u64, _P, _P]),
+    "ffbf_gen_tandem_repeats": (_i32, [_u64, _P, _u64, _u32, _u32, _u32, _u32, _u32, _i32]),
+    "ffbf_gen_sbox": (_i32, [_u64, _P]),
+    "ffbf_gen_u64": (_u64, [_u64, _u64, _u64]),
+}
+
+for _name, (_res, _args) in ABI.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def lib() -> ctypes.CDLL:
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data if a.size else 0
+
+
+def _check(status: int) -> None:
+    if status != OK:
+        msg = _lib.ffbf_last_error()
+        raise FFBFError(status, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    n = _c.c_int(0)
+    _check(_lib.ffbf_device_count(_c.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------- workloads
+def sbox_from_seed(seed: int) -> np.ndarray:
+    out = np.zeros(256, dtype=np.uint32)
+    _check(_lib.ffbf_gen_sbox(seed, _ptr(out)))
+    return out
+
+
+def gen_bytes(alphabet: int, seed: int, n: int, start: int = 0, threads: int = 0) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint8)
+    _check(_lib.ffbf_gen_bytes(alphabet, seed, start, n, _ptr(out), threads))
+    return out
+
+
+def gen_patterns(alphabet: int, seed: int, n: int, lmin: int, lmax: int, threads: int = 0):
+    """Pattern set as (bytes uint8[], offsets uint64[n+1])."""
+    off = np.empty(n + 1, dtype=np.uint64)
+    _check(_lib.ffbf_gen_pattern_offsets(seed, n, lmin, lmax, _ptr(off)))
+    total = int(off[-1])
+    b = np.empty(total, dtype=np.uint8)
+    _check(_lib.ffbf_gen_pattern_bytes(alphabet, seed, total, _ptr(b), threads))
+    return b, off
+
+
+def plant(seed: int, n_plants: int, text: np.ndarray, pat_bytes: np.ndarray, pat_off: np.ndarray):
+    """Copies n_plants seeded patterns into text (in place); returns (ids, offsets)."""
+    ids = np.empty(n_plants, dtype=np.uint32)
+    offs = np.empty(n_plants, dtype=np.uint64)
+    _check(
+        _lib.ffbf_gen_plants(
+            seed, n_plants, _ptr(text), text.size, _ptr(pat_bytes), _ptr(pat_off), pat_off.size - 1, _ptr(ids), _ptr(offs)
+        )
+    )
+    return ids, offs
+
+
+def tandem_repeats(seed: int, text: np.ndarray, block: int = 4096, min_len: int = 205, max_len: int = 614,
+                   min_period: int = 1, max_period: int = 6, threads: int = 0) -> None:
+    _check(_lib.ffbf_gen_tandem_repeats(seed, _ptr(text), text.size, block, min_len, max_len, min_period, max_period,
+                                        threads))
+
+
+def gen_u64(seed: int, tag: int, i: int) -> int:
+    return int(_lib.ffbf_gen_u64(seed, tag, i))
+
+
+def patterns_from_list(patterns: Sequence[bytes]):
+    off = np.zeros(len(patterns) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(p) for p in patterns], dtype=np.uint64) if patterns else []
+    b = np.frombuffer(b"".join(patterns), dtype=np.uint8).copy()
+    return b, off
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceBuffer:
+    """Plain device allocation on one HIP device (plumbing for tests/bench)."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        p = _c.c_void_p(0)
+        _check(_lib.ffbf_device_alloc(device, nbytes, _c.byref(p)))
+        self.ptr = p.value or 0
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, device: int = 0) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes, device)
+        buf.upload(a)
+        return buf
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError("upload past the end of the buffer")
+        _check(_lib.ffbf_device_upload(self.ptr + offset, _ptr(a), a.nbytes))
+
+    def download(self, nbytes: int | None = None, offset: int = 0) -> np.ndarray:
+        nbytes = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(nbytes, dtype=np.uint8)
+        _check(_lib.ffbf_device_download(_ptr(out), self.ptr + offset, nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.ffbf_device_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- matcher
+@dataclass
+class Result:
+    matches: np.ndarray  # MATCH_DTYPE, sorted by (pattern_id, offset)
+
+    @property
+    def pairs(self) -> list:
+        return list(zip(self.matches["pattern_id"].tolist(), self.matches["offset"].tolist()))
+
+
+class Matcher:
+    """One resident filter pair (F, R) and pattern set on one GPU."""
+
+    def __init__(self, window: int, k: int, log2_bits: int, sbox: Iterable[int], device: int = 0):
+        sb = np.asarray(list(sbox) if not isinstance(sbox, np.ndarray) else sbox, dtype=np.uint32)
+        if sb.shape != (256,):
+            raise ValueError("sbox must have 256 entries")
+        p = Params()
+        p.window, p.k, p.log2_bits, p.flags = window, k, log2_bits, 0
+        for i in range(256):
+            p.sbox[i] = int(sb[i])
+        h = _c.c_void_p(0)
+        _check(_lib.ffbf_create(_c.byref(p), device, _c.byref(h)))
+        self._h = h
+        self.window, self.k, self.log2_bits = window, k, log2_bits
+        self.n_words = 1 << (log2_bits - 5)
+        self.n_patterns = 0
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.ffbf_destroy(self._h)
+            self._h = _c.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- patterns / build
+    def set_patterns(self, pat_bytes: np.ndarray, pat_off: np.ndarray) -> None:
+        pat_bytes = np.ascontiguousarray(pat_bytes, dtype=np.uint8)
+        pat_off = np.ascontiguousarray(pat_off, dtype=np.uint64)
+        _check(_lib.ffbf_set_patterns(self._h, _ptr(pat_bytes), _ptr(pat_off), pat_off.size - 1))
+        self.n_patterns = pat_off.size - 1
+
+    def set_pattern_list(self, patterns: Sequence[bytes]) -> None:
+        self.set_patterns(*patterns_from_list(patterns))
+
+    def set_patterns_lines(self, text: bytes) -> None:
+        _check(_lib.ffbf_set_patterns_lines(self._h, text, len(text)))
+        self.n_patterns = text.count(b"\n") + (0 if (not text or text.endswith(b"\n")) else 1)
+
+    def build(self) -> None:
+        _check(_lib.ffbf_build(self._h))
+
+    def set_scan_mode(self, mode: int, fold_log2_bits: int = 0) -> None:
+        _check(_lib.ffbf_set_scan_mode(self._h, mode, fold_log2_bits))
+
+    # -- scan / pass / verify
+    def scan(self, text) -> int:
+        t = np.frombuffer(text, dtype=np.uint8) if isinstance(text, (bytes, bytearray)) else np.ascontiguousarray(text)
+        _check(_lib.ffbf_scan(self._h, _ptr(t), t.size))
+        return self.n_candidates()
+
+    def scan_device(self, ptr: int, n: int) -> int:
+        _check(_lib.ffbf_scan_device(self._h, ptr, n))
+        return self.n_candidates()
+
+    def run_device(self, ptr: int, n: int) -> int:
+        nm = _c.c_uint64(0)
+        _check(_lib.ffbf_run_device(self._h, ptr, n, _c.byref(nm)))
+        return nm.value
+
+    def pattern_pass(self) -> int:
+        n = _c.c_uint64(0)
+        _check(_lib.ffbf_pattern_pass(self._h, _c.byref(n)))
+        return n.value
+
+    def verify(self) -> int:
+        n = _c.c_uint64(0)
+        _check(_lib.ffbf_verify(self._h, _c.byref(n)))
+        return n.value
+
+    def match(self, text) -> Result:
+        t = np.frombuffer(text, dtype=np.uint8) if isinstance(text, (bytes, bytearray)) else np.ascontiguousarray(text)
+        out = _c.c_void_p(0)
+        n = _c.c_uint64(0)
+        _check(_lib.ffbf_match_text(self._h, _ptr(t), t.size, _c.byref(out), _c.byref(n)))
+        try:
+            if n.value:
+                raw = (_c.c_uint8 * (n.value * MATCH_DTYPE.itemsize)).from_address(out.value)
+                arr = np.frombuffer(raw, dtype=MATCH_DTYPE).copy()
+            else:
+                arr = np.zeros(0, dtype=MATCH_DTYPE)
+        finally:
+            _lib.ffbf_free_matches(out)
+        return Result(arr)
+
+    # -- results
+    def matches(self) -> np.ndarray:
+        n = _c.c_uint64(0)
+        st = _lib.ffbf_get_matches(self._h, None, 0, _c.byref(n))
+        if st not in (OK, ECAPACITY):
+            _check(st)
+        out = np.zeros(n.value, dtype=MATCH_DTYPE)
+        _check(_lib.ffbf_get_matches(self._h, _ptr(out), n.value, _c.byref(n)))
+        return out
+
+    def n_candidates(self) -> int:
+        n = _c.c_uint64(0)
+        st = _lib.ffbf_get_candidates(self._h, None, 0, _c.byref(n))
+        if st not in (OK, ECAPACITY):
+            _check(st)
+        return n.value
+
+    def candidates(self) -> np.ndarray:
+        n = self.n_candidates()
+        out = np.zeros(n, dtype=np.uint64)
+        m = _c.c_uint64(0)
+        _check(_lib.ffbf_get_candidates(self._h, _ptr(out), n, _c.byref(m)))
+        return out
+
+    def survivors(self) -> np.ndarray:
+        words = (self.n_patterns + 31) // 32
+        out = np.zeros(max(words, 1), dtype=np.uint32)
+        n = _c.c_uint64(0)
+        _check(_lib.ffbf_get_survivors(self._h, _ptr(out), words, _c.byref(n)))
+        return out[:words]
+
+    def filter(self, which: int = FILTER_FORWARD) -> np.ndarray:
+        out = np.zeros(self.n_words, dtype=np.uint32)
+        _check(_lib.ffbf_get_filter(self._h, which, _ptr(out), self.n_words))
+        return out
+
+    def filter_save(self) -> np.ndarray:
+        n = _c.c_uint64(0)
+        st = _lib.ffbf_filter_save(self._h, None, 0, _c.byref(n))
+        if st not in (OK, ECAPACITY):
+            _check(st)
+        out = np.zeros(n.value, dtype=np.uint32)
+        _check(_lib.ffbf_filter_save(self._h, _ptr(out), n.value, _c.byref(n)))
+        return out
+
+    def filter_load(self, deltas: np.ndarray) -> None:
+        d = np.ascontiguousarray(deltas, dtype=np.uint32)
+        _check(_lib.ffbf_filter_load(self._h, _ptr(d), d.size))
+
+    def timings(self) -> dict:
+        t = Timings()
+        _check(_lib.ffbf_get_timings(self._h, _c.byref(t)))
+        return t.as_dict()
+
+    def synchronize(self) -> None:
+        _check(_lib.ffbf_synchronize(self._h))
+
+    @property
+    def stream(self) -> int:
+        return _lib.ffbf_get_stream(self._h) or 0

@@ -1,0 +1,791 @@
+// ffbf_capi.cpp -- host context behind the C ABI (include/ffbf.h).
+//
+// The context owns every device buffer of the path: F and R (resident between
+// scans), the pattern set with its prefix hash pairs, the candidate list, the
+// survivor bitmap/list/index and the match list.  Stages are queued on one
+// HIP stream; counters come back through pinned memory.  Buffers that depend
+// on data sizes (candidates, matches, owned text) grow on demand: a stage
+// whose output overflowed is simply queued again with room for all of it,
+// which gives the same result because every stage is a pure function of its
+// inputs (filter bits are OR-idempotent).
+#include "ffbf.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ffbf_device.h"
+
+using ffbf::Cand;
+using ffbf::MatchRec;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(FFBF_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RET_IF(st)                \
+    do {                          \
+        int st_ = (st);           \
+        if (st_ != FFBF_OK) return st_; \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    uint64_t cap = 0;  // elements
+
+    int ensure(uint64_t n, const char *what)
+    {
+        if (n <= cap && p) return FFBF_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const uint64_t want = std::max<uint64_t>(n, 1);
+        hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(FFBF_ENOMEM, std::string("hipMalloc ") + what + " (" + std::to_string(want * sizeof(T)) +
+                                         " bytes): " + hipGetErrorString(e));
+        }
+        cap = want;
+        return FFBF_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+enum Ev { EV_B0, EV_B1, EV_S0, EV_S1, EV_S2, EV_P0, EV_P1, EV_V0, EV_V1, EV_N };
+
+uint32_t pow2_at_least(uint64_t n)
+{
+    uint32_t l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+}  // namespace
+
+struct ffbf_ctx {
+    ffbf_params params{};
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    uint32_t mask = 0;
+    uint64_t f_words = 0;
+
+    DevBuf<uint32_t> F, R, sbox, fold;
+    DevBuf<unsigned long long> counters;
+    unsigned long long *h_counters = nullptr;  // pinned
+
+    // scan mode
+    int mode_req = -1;
+    uint32_t fold_log2_req = 0;
+    uint32_t mode = ffbf::MODE_GLOBAL;
+    uint32_t fold_words = 0;
+
+    // pattern set
+    DevBuf<uint8_t> bytes;
+    DevBuf<uint64_t> off;
+    DevBuf<uint2> pat_hash;
+    DevBuf<uint32_t> bitmap, surv_ids, table, chain_next;
+    uint64_t n_pat = 0;
+    uint32_t table_size = 0;
+
+    // scan / results
+    const uint8_t *text = nullptr;
+    uint64_t text_n = 0;
+    DevBuf<uint8_t> owned_text;
+    DevBuf<Cand> cand;
+    DevBuf<MatchRec> matches;
+    uint64_t n_cand = 0, n_surv = 0, n_match = 0;
+    std::vector<MatchRec> sorted;
+    bool sorted_valid = false;
+
+    bool patterns = false, built = false, scanned = false, passed = false, verified = false;
+    hipEvent_t ev[EV_N] = {};
+    bool ev_recorded[EV_N] = {};
+};
+
+namespace {
+
+int use_device(ffbf_ctx *c)
+{
+    HIP_TRY(hipSetDevice(c->device));
+    return FFBF_OK;
+}
+
+int record(ffbf_ctx *c, int e)
+{
+    HIP_TRY(hipEventRecord(c->ev[e], c->stream));
+    c->ev_recorded[e] = true;
+    return FFBF_OK;
+}
+
+int sync_counters(ffbf_ctx *c)
+{
+    HIP_TRY(hipMemcpyAsync(c->h_counters, c->counters.p, ffbf::C_COUNT * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FFBF_OK;
+}
+
+int clear_counters(ffbf_ctx *c, int first, int count)
+{
+    HIP_TRY(ffbf::launch_fill32((uint32_t *)(c->counters.p + first), (uint64_t)count * 2u, 0u, c->stream));
+    return FFBF_OK;
+}
+
+// Picks how the scan pre-tests windows, after F changed.
+int choose_mode(ffbf_ctx *c)
+{
+    const uint32_t max_words = ffbf::max_fold_words(c->device);
+    const uint32_t k = c->params.k;
+    if (c->mode_req == ffbf::MODE_LDS || (c->mode_req < 0 && c->f_words <= max_words)) {
+        if (c->f_words > max_words) return fail(FFBF_EINVAL, "scan mode 2: filter does not fit in LDS");
+        if (c->f_words % 4u) {
+            // the LDS copy moves 16-byte chunks: fold onto itself padded is not
+            // possible, so use the global mode for tiny filters
+            c->mode = ffbf::MODE_GLOBAL;
+            return FFBF_OK;
+        }
+        c->mode = ffbf::MODE_LDS;
+        c->fold_words = (uint32_t)c->f_words;
+        return FFBF_OK;
+    }
+    if (c->mode_req == ffbf::MODE_GLOBAL) {
+        c->mode = ffbf::MODE_GLOBAL;
+        return FFBF_OK;
+    }
+    uint32_t words = max_words;
+    if (c->fold_log2_req) {
+        if (c->fold_log2_req < 7 || (1ull << (c->fold_log2_req - 5)) > max_words ||
+            (1ull << (c->fold_log2_req - 5)) > c->f_words)
+            return fail(FFBF_EINVAL, "fold size out of range");
+        words = 1u << (c->fold_log2_req - 5);
+    }
+    words = (uint32_t)std::min<uint64_t>(words, c->f_words);
+    RET_IF(c->fold.ensure(words, "fold"));
+    HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
+    c->fold_words = words;
+    if (c->mode_req == ffbf::MODE_FOLD) {
+        c->mode = ffbf::MODE_FOLD;
+        return FFBF_OK;
+    }
+    // automatic: fold fill decides whether the LDS pre-test rejects enough
+    std::vector<uint32_t> h(words);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->fold.p, words * 4ull, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    uint64_t ones = 0;
+    for (uint32_t w : h) ones += (uint64_t)__builtin_popcount(w);
+    double fill = (double)ones / ((double)words * 32.0), pass = 1.0;
+    for (uint32_t j = 0; j < k; j++) pass *= fill;
+    c->mode = pass < 0.25 ? ffbf::MODE_FOLD : ffbf::MODE_GLOBAL;
+    return FFBF_OK;
+}
+
+int enqueue_scan(ffbf_ctx *c)
+{
+    RET_IF(record(c, EV_S0));
+    HIP_TRY(ffbf::launch_fill32(c->R.p, c->f_words, 0u, c->stream));
+    RET_IF(clear_counters(c, ffbf::C_CAND, 1));
+    RET_IF(record(c, EV_S1));
+    ffbf::ScanArgs a{};
+    a.text = c->text;
+    a.n = c->text_n;
+    a.F = c->F.p;
+    a.R = c->R.p;
+    a.fold = c->mode == ffbf::MODE_LDS ? c->F.p : c->fold.p;
+    a.sbox = c->sbox.p;
+    a.cand = c->cand.p;
+    a.cand_cap = c->cand.cap;
+    a.counters = c->counters.p;
+    a.W = c->params.window;
+    a.k = c->params.k;
+    a.mask = c->mask;
+    a.fold_words = c->fold_words;
+    a.fold_mask = c->fold_words ? (uint32_t)((uint64_t)c->fold_words * 32u - 1u) : 0u;
+    HIP_TRY(ffbf::launch_scan(a, c->mode, c->n_cu, c->stream));
+    RET_IF(record(c, EV_S2));
+    return FFBF_OK;
+}
+
+int enqueue_pass(ffbf_ctx *c)
+{
+    RET_IF(record(c, EV_P0));
+    HIP_TRY(ffbf::launch_fill32(c->table.p, 2ull * c->table_size, 0u, c->stream));
+    RET_IF(clear_counters(c, ffbf::C_SURV, 1));
+    ffbf::PassArgs a{};
+    a.pat_hash = c->pat_hash.p;
+    a.off = c->off.p;
+    a.n = c->n_pat;
+    a.W = c->params.window;
+    a.k = c->params.k;
+    a.mask = c->mask;
+    a.R = c->R.p;
+    a.bitmap = c->bitmap.p;
+    a.surv_ids = c->surv_ids.p;
+    a.counters = c->counters.p;
+    a.table = c->table.p;
+    a.slot_chain = c->table.p + c->table_size;
+    a.chain_next = c->chain_next.p;
+    a.table_mask = c->table_size - 1u;
+    HIP_TRY(ffbf::launch_pattern_pass(a, c->n_cu, c->stream));
+    RET_IF(record(c, EV_P1));
+    return FFBF_OK;
+}
+
+int enqueue_verify(ffbf_ctx *c)
+{
+    RET_IF(record(c, EV_V0));
+    RET_IF(clear_counters(c, ffbf::C_MATCH, 1));
+    ffbf::VerifyArgs a{};
+    a.text = c->text;
+    a.n = c->text_n;
+    a.cand = c->cand.p;
+    a.cand_cap = c->cand.cap;
+    a.bytes = c->bytes.p;
+    a.off = c->off.p;
+    a.pat_hash = c->pat_hash.p;
+    a.table = c->table.p;
+    a.slot_chain = c->table.p + c->table_size;
+    a.chain_next = c->chain_next.p;
+    a.table_mask = c->table_size - 1u;
+    a.out = c->matches.p;
+    a.out_cap = c->matches.cap;
+    a.counters = c->counters.p;
+    HIP_TRY(ffbf::launch_verify(a, c->n_cu, c->stream));
+    RET_IF(record(c, EV_V1));
+    return FFBF_OK;
+}
+
+int check_ctx(ffbf_ctx *c)
+{
+    if (!c) return fail(FFBF_EINVAL, "null context");
+    return use_device(c);
+}
+
+int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n)
+{
+    if (!c->built) return fail(FFBF_ESTATE, "scan before build (or filter load)");
+    if (!dev_text && n) return fail(FFBF_EINVAL, "null text");
+    c->text = dev_text;
+    c->text_n = n;
+    const uint64_t want = std::max<uint64_t>(65536, n >> 10);
+    if (c->cand.cap < want) RET_IF(c->cand.ensure(want, "candidates"));
+    if (!c->matches.p) RET_IF(c->matches.ensure(65536, "matches"));
+    c->scanned = c->passed = c->verified = false;
+    c->sorted_valid = false;
+    return FFBF_OK;
+}
+
+// Scan, growing the candidate list until it holds every candidate.
+int do_scan(ffbf_ctx *c)
+{
+    RET_IF(enqueue_scan(c));
+    RET_IF(sync_counters(c));
+    if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
+        RET_IF(c->cand.ensure(c->h_counters[ffbf::C_CAND] + c->h_counters[ffbf::C_CAND] / 4 + 1024, "candidates"));
+        RET_IF(enqueue_scan(c));
+        RET_IF(sync_counters(c));
+    }
+    c->n_cand = c->h_counters[ffbf::C_CAND];
+    c->scanned = true;
+    c->passed = c->verified = false;
+    return FFBF_OK;
+}
+
+int do_verify(ffbf_ctx *c)
+{
+    RET_IF(enqueue_verify(c));
+    RET_IF(sync_counters(c));
+    if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
+        RET_IF(c->matches.ensure(c->h_counters[ffbf::C_MATCH] + c->h_counters[ffbf::C_MATCH] / 4 + 1024, "matches"));
+        RET_IF(enqueue_verify(c));
+        RET_IF(sync_counters(c));
+    }
+    c->n_match = c->h_counters[ffbf::C_MATCH];
+    c->verified = true;
+    c->sorted_valid = false;
+    return FFBF_OK;
+}
+
+int load_sorted_matches(ffbf_ctx *c)
+{
+    if (c->sorted_valid) return FFBF_OK;
+    c->sorted.resize(c->n_match);
+    if (c->n_match) {
+        HIP_TRY(hipMemcpyAsync(c->sorted.data(), c->matches.p, c->n_match * sizeof(MatchRec), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    std::sort(c->sorted.begin(), c->sorted.end(), [](const MatchRec &a, const MatchRec &b) {
+        return a.id != b.id ? a.id < b.id : a.offset < b.offset;
+    });
+    c->sorted_valid = true;
+    return FFBF_OK;
+}
+
+float elapsed(ffbf_ctx *c, int a, int b)
+{
+    if (!c->ev_recorded[a] || !c->ev_recorded[b]) return 0.0f;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ffbf_last_error(void) { return g_err.c_str(); }
+
+const char *ffbf_version(void) { return "ffbf-amd 0.1 (gfx950)"; }
+
+int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out)
+{
+    if (!params || !out) return fail(FFBF_EINVAL, "null argument");
+    *out = nullptr;
+    if (params->window < 1 || params->window > 64) return fail(FFBF_EINVAL, "window must be 1..64");
+    if (params->k < 1 || params->k > 5) return fail(FFBF_EINVAL, "k must be 1..5");
+    if (params->log2_bits < 10 || params->log2_bits > 32) return fail(FFBF_EINVAL, "log2_bits must be 10..32");
+    if (params->flags != 0) return fail(FFBF_EINVAL, "flags must be 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(FFBF_EDEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return fail(FFBF_EINVAL, "device out of range");
+    ffbf_ctx *c = new (std::nothrow) ffbf_ctx();
+    if (!c) return fail(FFBF_ENOMEM, "context allocation");
+    c->params = *params;
+    c->device = device;
+    int st = FFBF_OK;
+    auto init = [&]() -> int {
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (int i = 0; i < EV_N; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
+        c->mask = params->log2_bits >= 32 ? 0xFFFFFFFFu : ((1u << params->log2_bits) - 1u);
+        c->f_words = 1ull << (params->log2_bits - 5);
+        RET_IF(c->F.ensure(c->f_words, "forward filter"));
+        RET_IF(c->R.ensure(c->f_words, "reverse filter"));
+        RET_IF(c->sbox.ensure(256, "sbox"));
+        RET_IF(c->counters.ensure(ffbf::C_COUNT, "counters"));
+        HIP_TRY(hipHostMalloc((void **)&c->h_counters, ffbf::C_COUNT * sizeof(unsigned long long),
+                              hipHostMallocDefault));
+        HIP_TRY(hipMemcpyAsync(c->sbox.p, params->sbox, 256 * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
+        HIP_TRY(ffbf::launch_fill32(c->R.p, c->f_words, 0u, c->stream));
+        RET_IF(clear_counters(c, 0, ffbf::C_COUNT));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return FFBF_OK;
+    };
+    st = init();
+    if (st != FFBF_OK) {
+        std::string msg = g_err;
+        ffbf_destroy(c);
+        g_err = msg;
+        return st;
+    }
+    *out = c;
+    return FFBF_OK;
+}
+
+void ffbf_destroy(ffbf_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->F.release();
+    c->R.release();
+    c->sbox.release();
+    c->fold.release();
+    c->counters.release();
+    c->bytes.release();
+    c->off.release();
+    c->pat_hash.release();
+    c->bitmap.release();
+    c->surv_ids.release();
+    c->table.release();
+    c->chain_next.release();
+    c->owned_text.release();
+    c->cand.release();
+    c->matches.release();
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    for (int i = 0; i < EV_N; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    (void)hipGetLastError();
+    delete c;
+}
+
+int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets, uint64_t n)
+{
+    RET_IF(check_ctx(c));
+    if (!offsets) return fail(FFBF_EINVAL, "null offsets");
+    if (n > 0xFFFFFFFEull) return fail(FFBF_EINVAL, "at most 2^32-2 patterns");
+    if (offsets[0] != 0) return fail(FFBF_EINVAL, "offsets[0] must be 0");
+    for (uint64_t i = 0; i < n; i++)
+        if (offsets[i + 1] < offsets[i]) return fail(FFBF_EINVAL, "offsets must be non-decreasing");
+    const uint64_t total = offsets[n];
+    if (total && !bytes) return fail(FFBF_EINVAL, "null pattern bytes");
+    RET_IF(c->bytes.ensure(total, "pattern bytes"));
+    RET_IF(c->off.ensure(n + 1, "pattern offsets"));
+    RET_IF(c->pat_hash.ensure(n, "pattern hashes"));
+    RET_IF(c->bitmap.ensure((n + 31) / 32, "survivor bitmap"));
+    RET_IF(c->surv_ids.ensure(n, "survivor ids"));
+    RET_IF(c->chain_next.ensure(n, "survivor chains"));
+    const uint32_t tl = std::max<uint32_t>(pow2_at_least(2 * n), 10);
+    c->table_size = 1u << tl;
+    RET_IF(c->table.ensure(2ull * c->table_size, "survivor index"));
+    if (total) HIP_TRY(hipMemcpyAsync(c->bytes.p, bytes, total, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->off.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    RET_IF(clear_counters(c, ffbf::C_HASHED, 1));
+    // prefix hash pairs only (F untouched until ffbf_build)
+    HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->mask, nullptr,
+                               c->pat_hash.p, c->counters.p, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->n_pat = n;
+    c->patterns = true;
+    c->scanned = c->passed = c->verified = false;
+    return FFBF_OK;
+}
+
+int ffbf_set_patterns_lines(ffbf_ctx *c, const char *buf, uint64_t len)
+{
+    if (!buf && len) return fail(FFBF_EINVAL, "null buffer");
+    std::vector<uint64_t> off;
+    std::vector<uint8_t> bytes;
+    bytes.reserve(len);
+    off.push_back(0);
+    uint64_t start = 0;
+    for (uint64_t i = 0; i < len; i++) {
+        if (buf[i] == '\n') {
+            bytes.insert(bytes.end(), buf + start, buf + i);
+            off.push_back(bytes.size());
+            start = i + 1;
+        }
+    }
+    if (start < len) {
+        bytes.insert(bytes.end(), buf + start, buf + len);
+        off.push_back(bytes.size());
+    }
+    return ffbf_set_patterns(c, bytes.data(), off.data(), off.size() - 1);
+}
+
+int ffbf_build(ffbf_ctx *c)
+{
+    RET_IF(check_ctx(c));
+    if (!c->patterns) return fail(FFBF_ESTATE, "build before patterns are loaded");
+    RET_IF(record(c, EV_B0));
+    HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
+    RET_IF(clear_counters(c, ffbf::C_HASHED, 1));
+    HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k, c->mask,
+                               c->F.p, c->pat_hash.p, c->counters.p, c->stream));
+    RET_IF(record(c, EV_B1));
+    RET_IF(choose_mode(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->built = true;
+    c->scanned = c->passed = c->verified = false;
+    return FFBF_OK;
+}
+
+int ffbf_set_scan_mode(ffbf_ctx *c, int mode, uint32_t fold_log2_bits)
+{
+    RET_IF(check_ctx(c));
+    if (mode < -1 || mode > 2) return fail(FFBF_EINVAL, "mode must be -1..2");
+    c->mode_req = mode;
+    c->fold_log2_req = fold_log2_bits;
+    if (c->built) {
+        RET_IF(choose_mode(c));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return FFBF_OK;
+}
+
+int ffbf_scan_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n)
+{
+    RET_IF(check_ctx(c));
+    RET_IF(set_text(c, dev_text, n));
+    return do_scan(c);
+}
+
+int ffbf_scan(ffbf_ctx *c, const uint8_t *text, uint64_t n)
+{
+    RET_IF(check_ctx(c));
+    if (!text && n) return fail(FFBF_EINVAL, "null text");
+    RET_IF(c->owned_text.ensure(n, "text"));
+    if (n) HIP_TRY(hipMemcpyAsync(c->owned_text.p, text, n, hipMemcpyHostToDevice, c->stream));
+    RET_IF(set_text(c, c->owned_text.p, n));
+    return do_scan(c);
+}
+
+int ffbf_pattern_pass(ffbf_ctx *c, uint64_t *n_survivors)
+{
+    RET_IF(check_ctx(c));
+    if (!c->scanned) return fail(FFBF_ESTATE, "pattern pass before scan");
+    RET_IF(enqueue_pass(c));
+    RET_IF(sync_counters(c));
+    c->n_surv = c->h_counters[ffbf::C_SURV];
+    c->passed = true;
+    c->verified = false;
+    if (n_survivors) *n_survivors = c->n_surv;
+    return FFBF_OK;
+}
+
+int ffbf_verify(ffbf_ctx *c, uint64_t *n_matches)
+{
+    RET_IF(check_ctx(c));
+    if (!c->passed) return fail(FFBF_ESTATE, "verify before pattern pass");
+    RET_IF(do_verify(c));
+    if (n_matches) *n_matches = c->n_match;
+    return FFBF_OK;
+}
+
+int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *n_matches)
+{
+    RET_IF(check_ctx(c));
+    RET_IF(set_text(c, dev_text, n));
+    RET_IF(enqueue_scan(c));
+    RET_IF(enqueue_pass(c));
+    RET_IF(enqueue_verify(c));
+    RET_IF(sync_counters(c));
+    if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
+        // rerun the three stages with room for every candidate
+        RET_IF(c->cand.ensure(c->h_counters[ffbf::C_CAND] + c->h_counters[ffbf::C_CAND] / 4 + 1024, "candidates"));
+        RET_IF(enqueue_scan(c));
+        RET_IF(enqueue_pass(c));
+        RET_IF(enqueue_verify(c));
+        RET_IF(sync_counters(c));
+    }
+    c->n_cand = c->h_counters[ffbf::C_CAND];
+    c->n_surv = c->h_counters[ffbf::C_SURV];
+    c->scanned = c->passed = true;
+    if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
+        RET_IF(c->matches.ensure(c->h_counters[ffbf::C_MATCH] + c->h_counters[ffbf::C_MATCH] / 4 + 1024, "matches"));
+        RET_IF(enqueue_verify(c));
+        RET_IF(sync_counters(c));
+    }
+    c->n_match = c->h_counters[ffbf::C_MATCH];
+    c->verified = true;
+    c->sorted_valid = false;
+    if (n_matches) *n_matches = c->n_match;
+    return FFBF_OK;
+}
+
+int ffbf_get_matches(ffbf_ctx *c, ffbf_match *out, uint64_t cap, uint64_t *n)
+{
+    RET_IF(check_ctx(c));
+    if (!c->verified) return fail(FFBF_ESTATE, "no verified matches");
+    if (n) *n = c->n_match;
+    if (cap < c->n_match) return fail(FFBF_ECAPACITY, "match buffer too small");
+    if (c->n_match && !out) return fail(FFBF_EINVAL, "null output");
+    RET_IF(load_sorted_matches(c));
+    if (c->n_match) std::memcpy(out, c->sorted.data(), c->n_match * sizeof(ffbf_match));
+    return FFBF_OK;
+}
+
+int ffbf_match_text(ffbf_ctx *c, const uint8_t *text, uint64_t n, ffbf_match **out, uint64_t *n_out)
+{
+    RET_IF(check_ctx(c));
+    if (!out || !n_out) return fail(FFBF_EINVAL, "null output");
+    *out = nullptr;
+    *n_out = 0;
+    if (!text && n) return fail(FFBF_EINVAL, "null text");
+    RET_IF(c->owned_text.ensure(n, "text"));
+    if (n) HIP_TRY(hipMemcpyAsync(c->owned_text.p, text, n, hipMemcpyHostToDevice, c->stream));
+    uint64_t nm = 0;
+    RET_IF(ffbf_run_device(c, c->owned_text.p, n, &nm));
+    RET_IF(load_sorted_matches(c));
+    ffbf_match *m = (ffbf_match *)std::malloc(std::max<uint64_t>(nm, 1) * sizeof(ffbf_match));
+    if (!m) return fail(FFBF_ENOMEM, "match list allocation");
+    if (nm) std::memcpy(m, c->sorted.data(), nm * sizeof(ffbf_match));
+    *out = m;
+    *n_out = nm;
+    return FFBF_OK;
+}
+
+void ffbf_free_matches(ffbf_match *m) { std::free(m); }
+
+int ffbf_get_filter(ffbf_ctx *c, int which, uint32_t *out, uint64_t n_words)
+{
+    RET_IF(check_ctx(c));
+    if (!out) return fail(FFBF_EINVAL, "null output");
+    if (n_words < c->f_words) return fail(FFBF_ECAPACITY, "filter buffer too small");
+    const uint32_t *src = nullptr;
+    if (which == FFBF_FILTER_FORWARD) src = c->F.p;
+    else if (which == FFBF_FILTER_REVERSE) src = c->R.p;
+    else return fail(FFBF_EINVAL, "which must be FFBF_FILTER_FORWARD or FFBF_FILTER_REVERSE");
+    HIP_TRY(hipMemcpyAsync(out, src, c->f_words * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FFBF_OK;
+}
+
+int ffbf_get_candidates(ffbf_ctx *c, uint64_t *out, uint64_t cap, uint64_t *n)
+{
+    RET_IF(check_ctx(c));
+    if (!c->scanned) return fail(FFBF_ESTATE, "no scan yet");
+    if (n) *n = c->n_cand;
+    if (cap < c->n_cand) return fail(FFBF_ECAPACITY, "candidate buffer too small");
+    if (c->n_cand && !out) return fail(FFBF_EINVAL, "null output");
+    std::vector<Cand> h(c->n_cand);
+    if (c->n_cand) {
+        HIP_TRY(hipMemcpyAsync(h.data(), c->cand.p, c->n_cand * sizeof(Cand), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    for (uint64_t i = 0; i < c->n_cand; i++) out[i] = h[i].offset;
+    std::sort(out, out + c->n_cand);
+    return FFBF_OK;
+}
+
+int ffbf_get_survivors(ffbf_ctx *c, uint32_t *bitmap, uint64_t n_words, uint64_t *n_survivors)
+{
+    RET_IF(check_ctx(c));
+    if (!c->passed) return fail(FFBF_ESTATE, "no pattern pass yet");
+    if (n_survivors) *n_survivors = c->n_surv;
+    const uint64_t need = (c->n_pat + 31) / 32;
+    if (n_words < need) return fail(FFBF_ECAPACITY, "bitmap buffer too small");
+    if (need && !bitmap) return fail(FFBF_EINVAL, "null output");
+    if (need) {
+        HIP_TRY(hipMemcpyAsync(bitmap, c->bitmap.p, need * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return FFBF_OK;
+}
+
+int ffbf_filter_save(ffbf_ctx *c, uint32_t *deltas, uint64_t cap, uint64_t *n)
+{
+    RET_IF(check_ctx(c));
+    if (!c->built) return fail(FFBF_ESTATE, "no filter built");
+    std::vector<uint32_t> h(c->f_words);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->F.p, c->f_words * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    uint64_t cnt = 0;
+    for (uint32_t w : h) cnt += (uint64_t)__builtin_popcount(w);
+    if (n) *n = cnt;
+    if (cap < cnt) return fail(FFBF_ECAPACITY, "delta buffer too small");
+    if (cnt && !deltas) return fail(FFBF_EINVAL, "null output");
+    uint32_t prev = 0;
+    uint64_t o = 0;
+    for (uint64_t w = 0; w < c->f_words; w++) {
+        for (uint32_t m = h[w]; m; m &= m - 1) {
+            const uint32_t bit = (uint32_t)(w * 32u) + (uint32_t)__builtin_ctz(m);
+            deltas[o++] = bit - prev;
+            prev = bit;
+        }
+    }
+    return FFBF_OK;
+}
+
+int ffbf_filter_load(ffbf_ctx *c, const uint32_t *deltas, uint64_t n)
+{
+    RET_IF(check_ctx(c));
+    if (!deltas && n) return fail(FFBF_EINVAL, "null deltas");
+    std::vector<uint32_t> h(c->f_words, 0u);
+    uint32_t prev = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        prev += deltas[i];
+        if ((prev & c->mask) != prev) return fail(FFBF_EINVAL, "bit index beyond the filter size");
+        h[prev >> 5] |= 1u << (prev & 31u);
+    }
+    HIP_TRY(hipMemcpyAsync(c->F.p, h.data(), c->f_words * 4, hipMemcpyHostToDevice, c->stream));
+    RET_IF(choose_mode(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->built = true;
+    c->scanned = c->passed = c->verified = false;
+    return FFBF_OK;
+}
+
+int ffbf_get_timings(ffbf_ctx *c, ffbf_timings *out)
+{
+    RET_IF(check_ctx(c));
+    if (!out) return fail(FFBF_EINVAL, "null output");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out->build_ms = elapsed(c, EV_B0, EV_B1);
+    out->scan_ms = elapsed(c, EV_S0, EV_S2);
+    out->scan_kernel_ms = elapsed(c, EV_S1, EV_S2);
+    out->pattern_pass_ms = elapsed(c, EV_P0, EV_P1);
+    out->verify_ms = elapsed(c, EV_V0, EV_V1);
+    out->scan_mode = c->mode;
+    out->fold_log2_bits = c->mode == ffbf::MODE_GLOBAL ? 0u : pow2_at_least((uint64_t)c->fold_words * 32u);
+    return FFBF_OK;
+}
+
+int ffbf_synchronize(ffbf_ctx *c)
+{
+    RET_IF(check_ctx(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FFBF_OK;
+}
+
+void *ffbf_get_stream(ffbf_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int ffbf_device_alloc(int device, uint64_t bytes, void **out)
+{
+    if (!out) return fail(FFBF_EINVAL, "null output");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(device));
+    hipError_t e = hipMalloc(out, std::max<uint64_t>(bytes, 1));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(FFBF_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return FFBF_OK;
+}
+
+int ffbf_device_free(void *p)
+{
+    if (p) HIP_TRY(hipFree(p));
+    return FFBF_OK;
+}
+
+int ffbf_device_upload(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return FFBF_OK;
+}
+
+int ffbf_device_download(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return FFBF_OK;
+}
+
+int ffbf_device_count(int *out)
+{
+    if (!out) return fail(FFBF_EINVAL, "null output");
+    *out = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *out = n;
+    return FFBF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// ffbf_device.h -- records and launchers shared by the HIP kernels
+// (ffbf_kernels.hip) and the host context (ffbf_capi.cpp).  Internal: not part
+// of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ffbf {
+
+// One candidate window: offset in the text and its hash pair (h1, h2), so the
+// pattern pass / verification never rehash the text.
+struct Cand {
+    uint64_t offset;
+    uint32_t h1, h2;
+};
+static_assert(sizeof(Cand) == 16, "Cand is 16 bytes");
+
+// Same layout as ffbf_match in ffbf.h.
+struct MatchRec {
+    uint32_t id, reserved;
+    uint64_t offset;
+};
+static_assert(sizeof(MatchRec) == 16, "MatchRec is 16 bytes");
+
+// Device counters (unsigned long long each).
+enum Counter : int { C_CAND = 0, C_SURV = 1, C_MATCH = 2, C_HASHED = 3, C_COUNT = 8 };
+
+// How the scan tests a window before touching F in HBM.
+enum ScanMode : uint32_t {
+    MODE_GLOBAL = 0,  // first forward bit read from F (L2/MALL), rest on survivors
+    MODE_FOLD = 1,    // OR-fold of F in LDS tested first (exact superset), then F
+    MODE_LDS = 2,     // the whole of F lives in LDS: the LDS test is exact
+};
+
+struct ScanArgs {
+    const uint8_t *text;
+    uint64_t n;
+    const uint32_t *F;
+    uint32_t *R;
+    const uint32_t *fold;  // fold words (MODE_FOLD / MODE_LDS), else unused
+    const uint32_t *sbox;
+    Cand *cand;
+    uint64_t cand_cap;
+    unsigned long long *counters;
+    uint32_t W, k, mask;
+    uint32_t fold_words;  // power of two
+    uint32_t fold_mask;   // fold_words*32 - 1
+};
+
+struct VerifyArgs {
+    const uint8_t *text;
+    uint64_t n;
+    const Cand *cand;
+    uint64_t cand_cap;
+    const uint8_t *bytes;
+    const uint64_t *off;
+    const uint2 *pat_hash;
+    const uint32_t *table;       // table_size slots: 0 empty, else representative id + 1
+    const uint32_t *slot_chain;  // table_size heads: further ids with the slot's key (id + 1)
+    const uint32_t *chain_next;  // per pattern: next id + 1 in its chain
+    uint32_t table_mask;
+    MatchRec *out;
+    uint64_t out_cap;
+    unsigned long long *counters;
+};
+
+struct PassArgs {
+    const uint2 *pat_hash;
+    const uint64_t *off;
+    uint64_t n;
+    uint32_t W, k, mask;
+    const uint32_t *R;
+    uint32_t *bitmap;   // ceil(n/32) words, fully rewritten
+    uint32_t *surv_ids; // compacted survivor ids
+    unsigned long long *counters;
+    uint32_t *table;
+    uint32_t *slot_chain;
+    uint32_t *chain_next;
+    uint32_t table_mask;
+};
+
+// Launchers (hipError_t of the launch).
+hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s);
+hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
+                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, unsigned long long *counters,
+                        hipStream_t s);
+hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
+hipError_t launch_scan(const ScanArgs &a, uint32_t mode, int n_cu, hipStream_t s);
+hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
+hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
+
+// Largest fold (in 32-bit words) the scan kernel can hold in LDS on this device.
+uint32_t max_fold_words(int device);
+// Whether a templated fast scan exists for this window length.
+bool scan_has_fast_path(uint32_t W);
+
+}  // namespace ffbf

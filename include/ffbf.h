@@ -1,0 +1,172 @@
+/*
+ * ffbf.h -- C ABI of the MI355X feed-forward Bloom filter (FFBF) matcher.
+ *
+ * The flow is the reference's: load a pattern set, build the forward filter
+ * F from each pattern's W-byte prefix window, scan a text (every offset whose
+ * k forward bits are set in F becomes a candidate and feeds k reverse bits
+ * forward into R), run the pattern pass against R, then verify the surviving
+ * (pattern, candidate) pairs exactly.  The result is the list of
+ * (pattern id, text offset) matches, pattern id being the 0-based position of
+ * the pattern in the loaded set (its line number for newline-separated input).
+ *
+ * Reference interfaces replaced (efficient/ffbf):
+ *   setup_bloom_search        src/gpgpu/greptest.cu:455-474   -> ffbf_build
+ *   executeGrepOverlap        src/gpgpu/greptest.cu:505-576   -> ffbf_scan / ffbf_scan_device
+ *   executePatternFiltering   src/gpgpu/greptest.cu:578-596   -> ffbf_pattern_pass
+ *   bvDump                    src/gpgpu/greptest.cu:341-351   -> ffbf_get_filter
+ *   positions bitmap copy-out src/gpgpu/greptest.cu:697-704   -> ffbf_get_candidates
+ *   patterns bitmap copy-out  src/gpgpu/greptest.cu:705-707   -> ffbf_get_survivors
+ *   grep -Ff hand-off         src/rabin-karp/README:49-52     -> ffbf_verify / ffbf_get_matches
+ *   Filter::operator<<        src/rabin-karp/ffbf.cpp:268-283 -> ffbf_set_patterns + ffbf_build
+ *   Filter::processFile       src/rabin-karp/ffbf.cpp:310-392 -> ffbf_scan
+ *   Filter::filterPhrases     src/rabin-karp/ffbf.cpp:394-485 -> ffbf_pattern_pass
+ *   Filter::save/loadFilterToFile  src/rabin-karp/ffbf.cpp:487-527 -> ffbf_filter_save / ffbf_filter_load
+ *
+ * Conventions: every call returns FFBF_OK (0) or a negative status and never
+ * exits or throws; ffbf_last_error() gives the calling thread's last message.
+ * A context owns all of its device memory (filters stay resident between
+ * scans).  Host pointers passed in are borrowed for the duration of the call;
+ * a device text passed to ffbf_scan_device is borrowed until the next scan.
+ * A context may be used by one thread at a time; distinct contexts are
+ * independent.  Filters are uint32 words, bit b in word b>>5 under mask
+ * 1u<<(b&31) (greptest.cu:80-89).
+ */
+#ifndef FFBF_H
+#define FFBF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFBF_OK 0
+#define FFBF_EINVAL (-1)    /* bad argument */
+#define FFBF_EDEVICE (-2)   /* HIP runtime or kernel failure */
+#define FFBF_ENOMEM (-3)    /* device or host allocation failed */
+#define FFBF_ESTATE (-4)    /* call out of order (e.g. scan before build) */
+#define FFBF_ECAPACITY (-5) /* caller buffer too small; required size reported */
+
+#define FFBF_FILTER_FORWARD 0 /* F: built from patterns */
+#define FFBF_FILTER_REVERSE 1 /* R: fed forward by the last scan */
+
+/* Method parameters.  The reference fixes them at compile time: HASH_LEN 19,
+ * BLOOMBITS 2^28, four hash functions and its sbox table
+ * (greptest.cu:50-54, 115-120; sbox.h:2-67). */
+typedef struct ffbf_params {
+    uint32_t window;    /* W, bytes per hashed window, 1..64 */
+    uint32_t k;         /* hash functions per filter, 1..5 */
+    uint32_t log2_bits; /* filter size 2^log2_bits bits, 10..32 */
+    uint32_t flags;     /* must be 0 */
+    uint32_t sbox[256]; /* substitution table of the rolling hash */
+} ffbf_params;
+
+typedef struct ffbf_match {
+    uint32_t pattern_id;
+    uint32_t reserved;
+    uint64_t offset;
+} ffbf_match;
+
+/* Device time in milliseconds of the most recent run of each stage,
+ * measured with HIP events on the context's stream. */
+typedef struct ffbf_timings {
+    float build_ms;        /* filter build (clear F, hash prefixes, set bits) */
+    float scan_ms;         /* clear R + scan with feed-forward */
+    float scan_kernel_ms;  /* the scan kernel alone */
+    float pattern_pass_ms; /* pattern pass (+ survivor index) */
+    float verify_ms;       /* exact verification */
+    uint32_t scan_mode;    /* 0 global first probe, 1 LDS fold prefilter, 2 whole filter in LDS */
+    uint32_t fold_log2_bits;
+} ffbf_timings;
+
+typedef struct ffbf_ctx ffbf_ctx;
+
+const char *ffbf_last_error(void);
+const char *ffbf_version(void);
+
+/* Creates a context on HIP device `device`, allocating F and R. */
+int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out);
+void ffbf_destroy(ffbf_ctx *ctx);
+
+/* Loads a pattern set: pattern i is bytes[offsets[i] .. offsets[i+1]);
+ * offsets has n_patterns+1 entries.  Copied to the device (resident).
+ * Patterns shorter than W are kept (their ids count) but never match,
+ * as ffbf.cpp:271-275 skips them. */
+int ffbf_set_patterns(ffbf_ctx *ctx, const uint8_t *bytes, const uint64_t *offsets, uint64_t n_patterns);
+
+/* Same, from newline-separated text (one pattern per line, id = 0-based line
+ * number as ffbf.cpp:404-446; a final line without '\n' counts). */
+int ffbf_set_patterns_lines(ffbf_ctx *ctx, const char *buf, uint64_t len);
+
+/* Builds F from the loaded patterns (greptest.cu:455-474, :96-123). */
+int ffbf_build(ffbf_ctx *ctx);
+
+/* Scans a host text: copies it to device memory owned by the context, clears
+ * R, scans every offset in [0, n-W] (executeGrepOverlap, greptest.cu:505-576). */
+int ffbf_scan(ffbf_ctx *ctx, const uint8_t *text, uint64_t n);
+
+/* Scans a text already in device memory of the context's device. */
+int ffbf_scan_device(ffbf_ctx *ctx, const uint8_t *dev_text, uint64_t n);
+
+/* Scan, pattern pass and verification of a device-resident text queued back
+ * to back on the context's stream with one synchronisation at the end (the
+ * whole of greptest.cu:665-677 plus the exact step).  n_matches may be NULL;
+ * the matches are then read with ffbf_get_matches. */
+int ffbf_run_device(ffbf_ctx *ctx, const uint8_t *dev_text, uint64_t n, uint64_t *n_matches);
+
+/* Pattern pass against R (executePatternFiltering, greptest.cu:578-596).
+ * n_survivors may be NULL. */
+int ffbf_pattern_pass(ffbf_ctx *ctx, uint64_t *n_survivors);
+
+/* Exact verification of candidates x survivors; n_matches may be NULL. */
+int ffbf_verify(ffbf_ctx *ctx, uint64_t *n_matches);
+
+/* Copies the verified matches, sorted by (pattern id, offset).  *n receives
+ * the total; returns FFBF_ECAPACITY if cap < total (nothing copied). */
+int ffbf_get_matches(ffbf_ctx *ctx, ffbf_match *out, uint64_t cap, uint64_t *n);
+
+/* One call for the whole path on a host text: scan, pattern pass, verify.
+ * *out is allocated by the library (free with ffbf_free_matches). */
+int ffbf_match_text(ffbf_ctx *ctx, const uint8_t *text, uint64_t n, ffbf_match **out, uint64_t *n_out);
+void ffbf_free_matches(ffbf_match *matches);
+
+/* Filter dump (bvDump, greptest.cu:341-351): n_words must be >= 2^log2_bits/32. */
+int ffbf_get_filter(ffbf_ctx *ctx, int which, uint32_t *out, uint64_t n_words);
+
+/* Candidate offsets of the last scan, ascending (the reference's positions
+ * bitmap, greptest.cu:246, in list form). */
+int ffbf_get_candidates(ffbf_ctx *ctx, uint64_t *out, uint64_t cap, uint64_t *n);
+
+/* Survivor bitmap of the last pattern pass (greptest.cu:284): bit i of word
+ * i>>5 set iff pattern i survived; n_words >= ceil(n_patterns/32). */
+int ffbf_get_survivors(ffbf_ctx *ctx, uint32_t *bitmap, uint64_t n_words, uint64_t *n_survivors);
+
+/* F persisted as the reference's cache file body (ffbf.cpp:508-527): one
+ * uint32 delta per set bit.  ffbf_filter_load replaces F (ffbf.cpp:487-506). */
+int ffbf_filter_save(ffbf_ctx *ctx, uint32_t *deltas, uint64_t cap, uint64_t *n);
+int ffbf_filter_load(ffbf_ctx *ctx, const uint32_t *deltas, uint64_t n);
+
+/* How the scan tests windows before F in HBM (results never depend on it):
+ * mode -1 picks automatically after each build; 0 first probe in F;
+ * 1 OR-fold of F held in LDS, fold_log2_bits bits (0 = largest that fits);
+ * 2 whole filter in LDS (only when it fits).  Takes effect at the next scan. */
+int ffbf_set_scan_mode(ffbf_ctx *ctx, int mode, uint32_t fold_log2_bits);
+
+int ffbf_get_timings(ffbf_ctx *ctx, ffbf_timings *out);
+int ffbf_synchronize(ffbf_ctx *ctx);
+
+/* The context's HIP stream (a hipStream_t), for callers that order their own
+ * device work against the matcher's. */
+void *ffbf_get_stream(ffbf_ctx *ctx);
+
+/* Plain device-memory helpers for callers without their own HIP code. */
+int ffbf_device_alloc(int device, uint64_t bytes, void **out);
+int ffbf_device_free(void *ptr);
+int ffbf_device_upload(void *dst, const void *src, uint64_t bytes);
+int ffbf_device_download(void *dst, const void *src, uint64_t bytes);
+int ffbf_device_count(int *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
