@@ -142,6 +142,8 @@ def main() -> int:
     ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", type=int, default=-1, help="scan mode override (-1 auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (HBM counter reference) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -154,6 +156,14 @@ def main() -> int:
     import numpy as np
 
     import ffbf_amd as fa
+
+    if args.copy_calibration:
+        n = args.copy_calibration & ~15
+        src, dst = fa.DeviceBuffer(n), fa.DeviceBuffer(n)
+        for _ in range(3):
+            dst.copy_from(src, n)
+        print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return 0
 
     wl = WORKLOADS[args.workload]
     alpha, W, k, log2, n_pat, lmin, lmax, n_text, plants, seed = wl

@@ -41,7 +41,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY = -1, -2, -3, -4, -5
 FILTER_FORWARD, FILTER_REVERSE = 0, 1
-MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS = -1, 0, 1, 2
+MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS, MODE_GFOLD = -1, 0, 1, 2, 3
 ALPHA_ASCII, ALPHA_DNA_GC40 = 0, 1
 
 
@@ -72,6 +72,8 @@ class Timings(ctypes.Structure):
         ("verify_ms", ctypes.c_float),
         ("scan_mode", ctypes.c_uint32),
         ("fold_log2_bits", ctypes.c_uint32),
+        ("probes", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -108,6 +110,8 @@ ABI = {
     "ffbf_filter_save": (_i32, [_P, _P, _u64, _pu64]),
     "ffbf_filter_load": (_i32, [_P, _P, _u64]),
     "ffbf_set_scan_mode": (_i32, [_P, _i32, _u32]),
+    "ffbf_set_scan_tuning": (_i32, [_P, _u32, _u32]),
+    "ffbf_gather_bench": (_i32, [_i32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_get_timings": (_i32, [_P, _c.POINTER(Timings)]),
     "ffbf_synchronize": (_i32, [_P]),
     "ffbf_get_stream": (_P, [_P]),
@@ -116,6 +120,7 @@ ABI = {
     "ffbf_device_upload": (_i32, [_P, _P, _u64]),
     "ffbf_device_download": (_i32, [_P, _P, _u64]),
     "ffbf_device_count": (_i32, [_c.POINTER(_i32)]),
+    "ffbf_device_copy": (_i32, [_P, _P, _u64]),
     # workload generators (host only)
     "ffbf_gen_bytes": (_i32, [_u32, _u64, _u64, _u64, _P, _i32]),
     "ffbf_gen_pattern_offsets": (_i32, [_u64, _u64, _u32, _u32, _P]),
@@ -150,6 +155,14 @@ def device_count() -> int:
     n = _c.c_int(0)
     _check(_lib.ffbf_device_count(_c.byref(n)))
     return n.value
+
+
+def gather_bench(log2_bytes: int, steps: int = 64, device: int = 0):
+    """(ms, probes/s) of random uint32 gathers into a 2^log2_bytes array."""
+    ms = _c.c_float(0)
+    rate = _c.c_double(0)
+    _check(_lib.ffbf_gather_bench(device, log2_bytes, steps, _c.byref(ms), _c.byref(rate)))
+    return ms.value, rate.value
 
 
 # ---------------------------------------------------------------- workloads
@@ -233,6 +246,10 @@ class DeviceBuffer:
         _check(_lib.ffbf_device_download(_ptr(out), self.ptr + offset, nbytes))
         return out
 
+    def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
+        """Device-to-device copy with the library's plain copy kernel."""
+        _check(_lib.ffbf_device_copy(self.ptr, other.ptr, nbytes))
+
     def free(self) -> None:
         if self.ptr:
             _lib.ffbf_device_free(self.ptr)
@@ -309,6 +326,9 @@ class Matcher:
 
     def set_scan_mode(self, mode: int, fold_log2_bits: int = 0) -> None:
         _check(_lib.ffbf_set_scan_mode(self._h, mode, fold_log2_bits))
+
+    def set_scan_tuning(self, probes: int = 0, block: int = 0) -> None:
+        _check(_lib.ffbf_set_scan_tuning(self._h, probes, block))
 
     # -- scan / pass / verify
     def scan(self, text) -> int:

@@ -93,15 +93,18 @@ struct ffbf_ctx {
     uint32_t mask = 0;
     uint64_t f_words = 0;
 
-    DevBuf<uint32_t> F, R, sbox, fold;
+    DevBuf<uint32_t> F, R, sbox, fold, gfold;
     DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned
 
-    // scan mode
+    // scan mode and tuning
     int mode_req = -1;
     uint32_t fold_log2_req = 0;
+    uint32_t probes_req = 0, block_req = 0;
     uint32_t mode = ffbf::MODE_GLOBAL;
-    uint32_t fold_words = 0;
+    uint32_t pre_words = 0;  // prefilter size in words (F itself, LDS fold or global fold)
+    uint32_t probes = 1;     // LDS prefilter bits tested before queueing
+    double pre_fill = 0.0;   // fraction of prefilter bits set
 
     // pattern set
     DevBuf<uint8_t> bytes;
@@ -155,52 +158,112 @@ int clear_counters(ffbf_ctx *c, int first, int count)
     return FFBF_OK;
 }
 
-// Picks how the scan pre-tests windows, after F changed.
+double fill_of(ffbf_ctx *c, const uint32_t *dev, uint64_t words, int *st)
+{
+    std::vector<uint32_t> h(words);
+    *st = FFBF_OK;
+    if (hipMemcpyAsync(h.data(), dev, words * 4ull, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        *st = fail(FFBF_EDEVICE, std::string("prefilter readback: ") + hipGetErrorString(hipGetLastError()));
+        return 0.0;
+    }
+    uint64_t ones = 0;
+    for (uint32_t w : h) ones += (uint64_t)__builtin_popcount(w);
+    return (double)ones / ((double)words * 32.0);
+}
+
+// log2 bits of the largest fold that fits one XCD's L2 (4 MiB on MI355X;
+// the attribute is capped there in case it reports the sum over XCDs).
+uint32_t l2_fold_log2(ffbf_ctx *c)
+{
+    int l2 = 0;
+    if (hipDeviceGetAttribute(&l2, hipDeviceAttributeL2CacheSize, c->device) != hipSuccess || l2 <= 0) {
+        (void)hipGetLastError();
+        l2 = 4 << 20;
+    }
+    const uint64_t bytes = std::min<uint64_t>((uint64_t)l2, 4ull << 20);
+    uint32_t g = 10;
+    while ((1ull << (g + 1 - 3)) <= bytes && g + 1 < c->params.log2_bits) g++;
+    return g;
+}
+
+// Picks how the scan pre-tests windows, after F changed or a mode was
+// requested.  Automatic choice: the whole filter in LDS when it fits; else
+// the LDS fold when its fill leaves few windows for F; else F itself.
 int choose_mode(ffbf_ctx *c)
 {
     const uint32_t max_words = ffbf::max_fold_words(c->device);
     const uint32_t k = c->params.k;
+    int st = FFBF_OK;
+    uint32_t mode = 0;
     if (c->mode_req == ffbf::MODE_LDS || (c->mode_req < 0 && c->f_words <= max_words)) {
         if (c->f_words > max_words) return fail(FFBF_EINVAL, "scan mode 2: filter does not fit in LDS");
-        if (c->f_words % 4u) {
-            // the LDS copy moves 16-byte chunks: fold onto itself padded is not
-            // possible, so use the global mode for tiny filters
-            c->mode = ffbf::MODE_GLOBAL;
-            return FFBF_OK;
+        mode = ffbf::MODE_LDS;
+        c->pre_words = (uint32_t)c->f_words;
+        c->pre_fill = fill_of(c, c->F.p, c->f_words, &st);
+    } else if (c->mode_req == ffbf::MODE_GLOBAL) {
+        mode = ffbf::MODE_GLOBAL;
+        c->pre_words = (uint32_t)std::min<uint64_t>(c->f_words, 0xFFFFFFFFu);
+    } else if (c->mode_req == ffbf::MODE_GFOLD) {
+        const uint32_t l2 = c->fold_log2_req ? c->fold_log2_req : l2_fold_log2(c);
+        if (l2 < 10 || l2 > c->params.log2_bits || l2 > 31) return fail(FFBF_EINVAL, "global fold size out of range");
+        const uint32_t words = 1u << (l2 - 5);
+        RET_IF(c->gfold.ensure(words, "global fold"));
+        HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, words, c->stream));
+        mode = ffbf::MODE_GFOLD;
+        c->pre_words = words;
+    } else {
+        uint32_t words = max_words;
+        if (c->fold_log2_req) {
+            if (c->fold_log2_req < 7 || (1ull << (c->fold_log2_req - 5)) > max_words ||
+                (1ull << (c->fold_log2_req - 5)) > c->f_words)
+                return fail(FFBF_EINVAL, "fold size out of range");
+            words = 1u << (c->fold_log2_req - 5);
         }
-        c->mode = ffbf::MODE_LDS;
-        c->fold_words = (uint32_t)c->f_words;
-        return FFBF_OK;
+        words = (uint32_t)std::min<uint64_t>(words, c->f_words);
+        RET_IF(c->fold.ensure(words, "fold"));
+        HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
+        c->pre_words = words;
+        c->pre_fill = fill_of(c, c->fold.p, words, &st);
+        RET_IF(st);
+        // the LDS fold pays while few windows get past its first two bits;
+        // otherwise probe F directly if it fits in one XCD's L2, else a fold
+        // of F sized to L2
+        const double pass2 = c->pre_fill * (k > 1 ? c->pre_fill : 1.0);
+        if (c->mode_req == ffbf::MODE_FOLD || pass2 < 0.3) {
+            mode = ffbf::MODE_FOLD;
+        } else {
+            const uint32_t gl2 = l2_fold_log2(c);
+            if (c->f_words * 4ull <= (4ull << 20)) {
+                mode = ffbf::MODE_GLOBAL;
+                c->pre_words = (uint32_t)c->f_words;
+            } else {
+                const uint32_t words = 1u << (gl2 - 5);
+                RET_IF(c->gfold.ensure(words, "global fold"));
+                HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, words, c->stream));
+                mode = ffbf::MODE_GFOLD;
+                c->pre_words = words;
+            }
+        }
     }
-    if (c->mode_req == ffbf::MODE_GLOBAL) {
-        c->mode = ffbf::MODE_GLOBAL;
-        return FFBF_OK;
+    RET_IF(st);
+    c->mode = mode;
+    // LDS prefilter bits tested up front: enough to leave few windows queued
+    if (c->probes_req) {
+        c->probes = std::min(c->probes_req, k);
+    } else if (mode == ffbf::MODE_FOLD || mode == ffbf::MODE_LDS) {
+        c->probes = c->pre_fill < 0.15 ? 1u : 2u;
+        c->probes = std::min(c->probes, k);
+    } else {
+        c->probes = 1;
     }
-    uint32_t words = max_words;
-    if (c->fold_log2_req) {
-        if (c->fold_log2_req < 7 || (1ull << (c->fold_log2_req - 5)) > max_words ||
-            (1ull << (c->fold_log2_req - 5)) > c->f_words)
-            return fail(FFBF_EINVAL, "fold size out of range");
-        words = 1u << (c->fold_log2_req - 5);
-    }
-    words = (uint32_t)std::min<uint64_t>(words, c->f_words);
-    RET_IF(c->fold.ensure(words, "fold"));
-    HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
-    c->fold_words = words;
-    if (c->mode_req == ffbf::MODE_FOLD) {
-        c->mode = ffbf::MODE_FOLD;
-        return FFBF_OK;
-    }
-    // automatic: fold fill decides whether the LDS pre-test rejects enough
-    std::vector<uint32_t> h(words);
-    HIP_TRY(hipMemcpyAsync(h.data(), c->fold.p, words * 4ull, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    uint64_t ones = 0;
-    for (uint32_t w : h) ones += (uint64_t)__builtin_popcount(w);
-    double fill = (double)ones / ((double)words * 32.0), pass = 1.0;
-    for (uint32_t j = 0; j < k; j++) pass *= fill;
-    c->mode = pass < 0.25 ? ffbf::MODE_FOLD : ffbf::MODE_GLOBAL;
     return FFBF_OK;
+}
+
+uint32_t scan_block(ffbf_ctx *c)
+{
+    if (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) return c->block_req == 512u ? 512u : 1024u;
+    return 256u;
 }
 
 int enqueue_scan(ffbf_ctx *c)
@@ -214,7 +277,6 @@ int enqueue_scan(ffbf_ctx *c)
     a.n = c->text_n;
     a.F = c->F.p;
     a.R = c->R.p;
-    a.fold = c->mode == ffbf::MODE_LDS ? c->F.p : c->fold.p;
     a.sbox = c->sbox.p;
     a.cand = c->cand.p;
     a.cand_cap = c->cand.cap;
@@ -222,9 +284,17 @@ int enqueue_scan(ffbf_ctx *c)
     a.W = c->params.window;
     a.k = c->params.k;
     a.mask = c->mask;
-    a.fold_words = c->fold_words;
-    a.fold_mask = c->fold_words ? (uint32_t)((uint64_t)c->fold_words * 32u - 1u) : 0u;
-    HIP_TRY(ffbf::launch_scan(a, c->mode, c->n_cu, c->stream));
+    switch (c->mode) {
+    case ffbf::MODE_FOLD: a.pre = c->fold.p; break;
+    case ffbf::MODE_GFOLD: a.pre = c->gfold.p; break;
+    default: a.pre = c->F.p; break;
+    }
+    a.pre_words = c->pre_words;
+    a.pre_mask = c->mode == ffbf::MODE_GLOBAL ? c->mask : (uint32_t)((uint64_t)c->pre_words * 32u - 1u);
+    a.pre_log2w = pow2_at_least(c->pre_words);
+    a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
+    a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
+    HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
     RET_IF(record(c, EV_S2));
     return FFBF_OK;
 }
@@ -423,6 +493,7 @@ void ffbf_destroy(ffbf_ctx *c)
     c->R.release();
     c->sbox.release();
     c->fold.release();
+    c->gfold.release();
     c->counters.release();
     c->bytes.release();
     c->off.release();
@@ -516,13 +587,25 @@ int ffbf_build(ffbf_ctx *c)
 int ffbf_set_scan_mode(ffbf_ctx *c, int mode, uint32_t fold_log2_bits)
 {
     RET_IF(check_ctx(c));
-    if (mode < -1 || mode > 2) return fail(FFBF_EINVAL, "mode must be -1..2");
+    if (mode < -1 || mode > 3) return fail(FFBF_EINVAL, "mode must be -1..3");
     c->mode_req = mode;
     c->fold_log2_req = fold_log2_bits;
     if (c->built) {
         RET_IF(choose_mode(c));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
+    return FFBF_OK;
+}
+
+int ffbf_set_scan_tuning(ffbf_ctx *c, uint32_t probes, uint32_t block)
+{
+    RET_IF(check_ctx(c));
+    if (probes > c->params.k) return fail(FFBF_EINVAL, "probes must be 0..k");
+    if (block != 0 && block != 256 && block != 512 && block != 1024)
+        return fail(FFBF_EINVAL, "block must be 0, 256, 512 or 1024");
+    c->probes_req = probes;
+    c->block_req = block;
+    if (c->built) RET_IF(choose_mode(c));
     return FFBF_OK;
 }
 
@@ -730,7 +813,9 @@ int ffbf_get_timings(ffbf_ctx *c, ffbf_timings *out)
     out->pattern_pass_ms = elapsed(c, EV_P0, EV_P1);
     out->verify_ms = elapsed(c, EV_V0, EV_V1);
     out->scan_mode = c->mode;
-    out->fold_log2_bits = c->mode == ffbf::MODE_GLOBAL ? 0u : pow2_at_least((uint64_t)c->fold_words * 32u);
+    out->fold_log2_bits = c->mode == ffbf::MODE_GLOBAL ? 0u : pow2_at_least((uint64_t)c->pre_words * 32u);
+    out->probes = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
+    out->block = scan_block(c);
     return FFBF_OK;
 }
 
@@ -775,6 +860,15 @@ int ffbf_device_download(void *dst, const void *src, uint64_t bytes)
     return FFBF_OK;
 }
 
+int ffbf_device_copy(void *dst, const void *src, uint64_t bytes)
+{
+    if ((bytes % 16u) || ((uintptr_t)dst % 16u) || ((uintptr_t)src % 16u))
+        return fail(FFBF_EINVAL, "device copy needs 16-byte aligned pointers and size");
+    HIP_TRY(ffbf::launch_copy(dst, src, bytes, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    return FFBF_OK;
+}
+
 int ffbf_device_count(int *out)
 {
     if (!out) return fail(FFBF_EINVAL, "null output");
@@ -786,6 +880,39 @@ int ffbf_device_count(int *out)
     }
     *out = n;
     return FFBF_OK;
+}
+
+int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms, double *probes_per_s)
+{
+    if (log2_bytes < 4 || log2_bytes > 34 || steps == 0) return fail(FFBF_EINVAL, "log2_bytes must be 4..34, steps > 0");
+    HIP_TRY(hipSetDevice(device));
+    int n_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    const uint32_t blocks = (uint32_t)n_cu * 8u;
+    DevBuf<uint32_t> arr, out;
+    RET_IF(arr.ensure(1ull << (log2_bytes - 2), "gather array"));
+    RET_IF(out.ensure((uint64_t)blocks * 256u, "gather output"));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int st) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        arr.release();
+        out.release();
+        return st;
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return done(fail(FFBF_EDEVICE, "event creation"));
+    if (ffbf::launch_fill32(arr.p, 1ull << (log2_bytes - 2), 0x5A5A5A5Au, nullptr) != hipSuccess ||
+        ffbf::launch_gather(arr.p, log2_bytes - 2, steps, out.p, blocks, nullptr) != hipSuccess ||
+        hipEventRecord(e0, nullptr) != hipSuccess ||
+        ffbf::launch_gather(arr.p, log2_bytes - 2, steps, out.p, blocks, nullptr) != hipSuccess ||
+        hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        return done(fail(FFBF_EDEVICE, std::string("gather bench: ") + hipGetErrorString(hipGetLastError())));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t;
+    if (probes_per_s) *probes_per_s = (double)blocks * 256.0 * 16.0 * (double)steps / ((double)t * 1e-3);
+    return done(FFBF_OK);
 }
 
 }  // extern "C"

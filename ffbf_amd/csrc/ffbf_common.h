@@ -1,0 +1,137 @@
+// ffbf_common.h -- device helpers shared by the FFBF kernels (internal).
+//
+// Hash pair of the W-byte window at offset o (greptest.cu:106-114):
+//   h1 = XOR_j rotl(S[t[o+j]], W-1-j),  h2 = XOR_j rotl(S[t[o+j]], 3(W-1-j))
+// Forward bit indices (greptest.cu:115-120, 199-202; the fifth is the
+// commented-out (h1<<16)|(h2>>16) of :121-122):   h1, h2, h1+h2, h1+5h2
+// Reverse bit indices (greptest.cu:231-234, 273-276; the fifth is the
+// commented-out 11h1+h2 of :235):                 h1+7h2, h1+3h2, (h1<<1)+h2, (h1<<2)+h2
+// All mod 2^32, then masked to the filter size.  Bit b of a filter is bit
+// b&31 of word b>>5 (greptest.cu:80-89).
+#pragma once
+
+#include "ffbf_device.h"
+
+namespace ffbf {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t rotl(uint32_t x, uint32_t r) { return __builtin_rotateleft32(x, r); }
+
+__device__ __forceinline__ uint32_t fwd_idx(int j, uint32_t h1, uint32_t h2)
+{
+    switch (j) {
+    case 0: return h1;
+    case 1: return h2;
+    case 2: return h1 + h2;
+    case 3: return h1 + 5u * h2;
+    default: return (h1 << 16) | (h2 >> 16);
+    }
+}
+
+__device__ __forceinline__ uint32_t rev_idx(int j, uint32_t h1, uint32_t h2)
+{
+    switch (j) {
+    case 0: return h1 + 7u * h2;
+    case 1: return h1 + 3u * h2;
+    case 2: return (h1 << 1) + h2;
+    case 3: return (h1 << 2) + h2;
+    default: return h1 * 11u + h2;
+    }
+}
+
+__device__ __forceinline__ uint32_t bit_of(const uint32_t *v, uint32_t b) { return (v[b >> 5] >> (b & 31u)) & 1u; }
+
+// Bit x (taken modulo 32*2^log2_words) of a filter held in LDS: word index by
+// one bit-field extract, bit by another (its offset operand uses x's low 5 bits).
+__device__ __forceinline__ uint32_t lds_bit(const uint32_t *s, uint32_t x, uint32_t log2_words)
+{
+    const uint32_t w = s[__builtin_amdgcn_ubfe(x, 5u, log2_words)];
+    return __builtin_amdgcn_ubfe(w, x & 31u, 1u);
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Orders this wave's LDS traffic across its lanes: writes before the call are
+// seen by reads after it, from any lane of the same wave.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Hash-table slot of a prefix-window hash pair (verification index).
+__device__ __forceinline__ uint32_t key_slot(uint32_t h1, uint32_t h2) { return mix32(h1 ^ rotl(h2, 13) ^ 0x9E3779B9u); }
+
+// Feed-forward: set the k reverse bits of every lane with `hit` in R.  Lanes
+// of the wave that target the same word are merged into one atomicOr (the
+// reference issues one atomicOr per bit, greptest.cu:86-89, 236-239).  Must be
+// called by all lanes of the wave.
+__device__ __forceinline__ void feed_forward(bool hit, uint32_t h1, uint32_t h2, uint32_t k, uint32_t mask, uint32_t *R)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        if (j >= (int)k) break;
+        const uint32_t idx = rev_idx(j, h1, h2) & mask;
+        const uint32_t w = idx >> 5, bit = 1u << (idx & 31u);
+        uint64_t act = __ballot(hit);
+        while (act) {
+            const int leader = __builtin_ctzll(act);
+            const uint32_t w0 = __builtin_amdgcn_readlane(w, leader);
+            const uint64_t same = __ballot(hit && w == w0) & act;
+            uint32_t bits = 0;
+            for (uint64_t m = same; m; m &= m - 1) bits |= __builtin_amdgcn_readlane(bit, __builtin_ctzll(m));
+            if (lane == leader) atomicOr(&R[w0], bits);
+            act &= ~same;
+        }
+    }
+}
+
+// Appends (offset, h1, h2) of every lane with `hit`: one atomicAdd per wave,
+// slots by lane rank.  Must be called by all lanes of the wave.
+__device__ __forceinline__ void append_cand(bool hit, uint64_t off, uint32_t h1, uint32_t h2, Cand *cand, uint64_t cap,
+                                            unsigned long long *counter)
+{
+    const uint64_t hb = __ballot(hit);
+    if (!hb) return;
+    const int leader = __builtin_ctzll(hb);
+    unsigned long long first = 0;
+    if (lane_id() == leader) first = atomicAdd(counter, (unsigned long long)__builtin_popcountll(hb));
+    first = readlane64(first, leader);
+    if (hit) {
+        const uint64_t slot = first + lanes_below(hb);
+        if (slot < cap) {
+            Cand c;
+            c.offset = off;
+            c.h1 = h1;
+            c.h2 = h2;
+            cand[slot] = c;
+        }
+    }
+}
+
+}  // namespace ffbf

@@ -26,11 +26,12 @@ static_assert(sizeof(MatchRec) == 16, "MatchRec is 16 bytes");
 // Device counters (unsigned long long each).
 enum Counter : int { C_CAND = 0, C_SURV = 1, C_MATCH = 2, C_HASHED = 3, C_COUNT = 8 };
 
-// How the scan tests a window before touching F in HBM.
+// How the scan tests a window before the full forward probe.
 enum ScanMode : uint32_t {
-    MODE_GLOBAL = 0,  // first forward bit read from F (L2/MALL), rest on survivors
-    MODE_FOLD = 1,    // OR-fold of F in LDS tested first (exact superset), then F
-    MODE_LDS = 2,     // the whole of F lives in LDS: the LDS test is exact
+    MODE_GLOBAL = 0,  // first forward bit read from F itself (L2/MALL), the rest for survivors
+    MODE_FOLD = 1,    // OR-fold of F held in LDS tested first (exact superset), then F
+    MODE_LDS = 2,     // the whole of F held in LDS: the LDS test is exact
+    MODE_GFOLD = 3,   // OR-fold of F in global memory, small enough to stay in L2, then F
 };
 
 struct ScanArgs {
@@ -38,14 +39,17 @@ struct ScanArgs {
     uint64_t n;
     const uint32_t *F;
     uint32_t *R;
-    const uint32_t *fold;  // fold words (MODE_FOLD / MODE_LDS), else unused
+    const uint32_t *pre;  // prefilter words: F itself or a fold of it
     const uint32_t *sbox;
     Cand *cand;
     uint64_t cand_cap;
     unsigned long long *counters;
     uint32_t W, k, mask;
-    uint32_t fold_words;  // power of two
-    uint32_t fold_mask;   // fold_words*32 - 1
+    uint32_t pre_words;  // power of two (>= 4 when copied to LDS)
+    uint32_t pre_mask;   // pre_words*32 - 1
+    uint32_t pre_log2w;  // log2(pre_words)
+    uint32_t P;          // forward bits tested against the prefilter before queueing (1..k; 1 for global)
+    uint32_t pre_exact;  // prefilter is F itself: no further F probes
 };
 
 struct VerifyArgs {
@@ -82,11 +86,14 @@ struct PassArgs {
 
 // Launchers (hipError_t of the launch).
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
                         uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, unsigned long long *counters,
                         hipStream_t s);
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
-hipError_t launch_scan(const ScanArgs &a, uint32_t mode, int n_cu, hipStream_t s);
+hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s);
+hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
+                         hipStream_t s);
 hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
 

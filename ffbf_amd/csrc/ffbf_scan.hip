@@ -1,0 +1,180 @@
+// ffbf_scan.hip -- text scan with feed-forward (GrepKernel, greptest.cu:125-251)
+// for gfx950, plus the random-gather microbenchmark that bounds it.
+//
+// Work layout: the text is cut into 16-byte units aligned to 16 bytes in
+// memory; one lane owns the 16 windows that start in one unit, so a wave
+// covers 1 KiB of window starts per iteration and persistent waves stride
+// over the text.  A lane loads its unit plus the next (W-1)/16 units with
+// 128-bit non-temporal loads (the next iteration's units are loaded while the
+// current ones are hashed), looks every byte up once in an LDS copy of the
+// sbox, and computes its 16 hash pairs as one full window plus 15 rolling
+// slides (hash_rot_sbox_2.h:34-51).
+//
+// Each window is first tested against a prefilter: P forward bits of either
+// F itself or an OR-fold of F (a fold bit is the OR of all F bits congruent
+// to it, so a window whose forward bits are all set in F has them set in the
+// fold: the test never rejects a true candidate).  The prefilter lives in
+// LDS (fold or whole filter) or in global memory (F, or a fold small enough
+// to stay in L2).  Windows that pass are pushed onto a per-wave LDS queue;
+// 64 at a time, one per lane, they finish their forward probes, feed their
+// reverse bits forward into R and are appended to the candidate list.
+#include "ffbf_scan_common.h"
+
+namespace ffbf {
+
+namespace {
+
+template <int WT, bool LDSPRE, int BLOCK, int P>
+__global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
+{
+    using QG = QueueGeom<BLOCK>;
+    extern __shared__ __align__(16) uint32_t s_pre[];
+    __shared__ uint32_t s_sbox[256];
+    __shared__ uint4 s_q[BLOCK / 64][QG::cap];
+
+    for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) s_sbox[i] = a.sbox[i];
+    if constexpr (LDSPRE) {
+        const uint4 *src = (const uint4 *)a.pre;
+        uint4 *dst = (uint4 *)s_pre;
+        for (uint32_t i = threadIdx.x; i < a.pre_words / 4u; i += BLOCK) dst[i] = src[i];
+    }
+    __syncthreads();
+
+    const uint32_t W = WT ? (uint32_t)WT : a.W;
+    const int wv = (int)(threadIdx.x >> 6);
+    const uint64_t lead = (uint64_t)((uintptr_t)a.text & 15u);
+    const uint8_t *base = a.text - lead;
+    const uint64_t npos = a.n >= W ? a.n - W + 1 : 0;
+    const uint64_t units = (lead + npos + 15u) >> 4;
+    const uint64_t iters = (units + 63u) >> 6;
+    const uint64_t n_waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    uint4 *q = s_q[wv];
+    uint32_t qn = 0;  // wave-uniform queue depth
+
+    constexpr int NCH = WT > 0 ? Chunks<(WT > 0 ? WT : 1)>::NCH : 1;
+    WaveChunks<NCH> cur;
+    uint64_t it = (uint64_t)blockIdx.x * (BLOCK / 64) + wv;
+    if constexpr (WT > 0) {
+        if (it < iters) cur.load(base, lead, a.n, it << 6);
+    }
+
+    for (; it < iters; it += n_waves) {
+        const uint64_t u = (it << 6) + (uint64_t)lane_id();
+        uint32_t h1[kUnit], h2[kUnit];
+        if constexpr (WT > 0) {
+            // issue the next iteration's loads before hashing this one
+            WaveChunks<NCH> nxt;
+            const uint64_t nit = it + n_waves;
+            if (nit < iters) nxt.load(base, lead, a.n, nit << 6);
+            uint4 ch[NCH];
+            cur.spread(ch);
+            hash_chunks<(WT > 0 ? WT : 1)>(ch, s_sbox, h1, h2);
+            cur = nxt;
+        } else {
+            hash_unit_generic(base, lead, a.n, u, W, s_sbox, h1, h2);
+        }
+
+        // window offset of slot b is 16u + b - lead
+        test_and_queue<LDSPRE, BLOCK, P>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos, 0, q, qn);
+    }
+    if (qn) {
+        wave_sync();
+        drain<LDSPRE, P>(q, qn, a, s_pre);
+    }
+}
+
+template <int WT, bool LDSPRE, int BLOCK, int P>
+hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
+{
+    const size_t dyn = LDSPRE ? (size_t)a.pre_words * 4u : 0;
+    if (dyn > 0) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan<WT, LDSPRE, BLOCK, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<WT, LDSPRE, BLOCK, P>, BLOCK, dyn);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    const uint64_t W = WT ? (uint64_t)WT : a.W;
+    const uint64_t npos = a.n >= W ? a.n - W + 1 : 0;
+    const uint64_t iters = (npos + 16u + 1023u) / 1024u;
+    uint64_t grid = (uint64_t)n_cu * (uint64_t)per_cu;
+    const uint64_t need = (iters + (BLOCK / 64) - 1) / (BLOCK / 64);
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL((k_scan<WT, LDSPRE, BLOCK, P>), dim3((uint32_t)grid), dim3(BLOCK), dyn, s, a);
+    return hipGetLastError();
+}
+
+// LDS prefilter: 1024-thread blocks (512 where a smaller LDS forces it), P
+// bits tested up front; global prefilter: 256-thread blocks, one bit.
+template <int WT>
+hipError_t launch_scan_w(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s)
+{
+    const bool lds = mode == MODE_FOLD || mode == MODE_LDS;
+    if (lds) {
+        const uint32_t P = a.P < 1u ? 1u : (a.P > 3u ? 3u : a.P);
+        if (block == 512) {
+            if (P == 1) return launch_scan_t<WT, true, 512, 1>(a, n_cu, s);
+            if (P == 2) return launch_scan_t<WT, true, 512, 2>(a, n_cu, s);
+            return launch_scan_t<WT, true, 512, 3>(a, n_cu, s);
+        }
+        if (P == 1) return launch_scan_t<WT, true, 1024, 1>(a, n_cu, s);
+        if (P == 2) return launch_scan_t<WT, true, 1024, 2>(a, n_cu, s);
+        return launch_scan_t<WT, true, 1024, 3>(a, n_cu, s);
+    }
+    return launch_scan_t<WT, false, 256, 1>(a, n_cu, s);
+}
+
+// Random dword gathers into a 2^log2_words-word array: 16 independent loads
+// per lane per step, addresses from a counter hash.
+__global__ __launch_bounds__(256) void k_gather(const uint32_t *arr, uint32_t mask, uint32_t steps, uint32_t *out)
+{
+    uint32_t x = mix32((uint32_t)(blockIdx.x * 256u + threadIdx.x) * 0x9E3779B9u + 1u);
+    uint32_t acc = 0;
+    for (uint32_t s = 0; s < steps; s++) {
+        uint32_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = arr[mix32(x + (uint32_t)j * 0x61C88647u) & mask];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc ^= v[j];
+        x = mix32(x ^ acc);
+    }
+    out[blockIdx.x * 256u + threadIdx.x] = acc;
+}
+
+}  // namespace
+
+bool scan_has_fast_path(uint32_t W) { return W == 19 || W == 24 || W == 32; }
+
+uint32_t max_fold_words(int device)
+{
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds <= 0)
+        lds = 64 * 1024;
+    // static LDS of the 512-thread scan: sbox 1 KiB + 8 queues of 2 KiB
+    const int static_lds = 1024 + 8 * QueueGeom<512>::cap * 16;
+    uint32_t words = 1u << 15;  // 128 KiB
+    while (words > 256u && (int)(words * 4u) + static_lds > lds) words >>= 1;
+    return words;
+}
+
+hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s)
+{
+    switch (a.W) {
+    case 19: return launch_scan_w<19>(a, mode, block, n_cu, s);
+    case 24: return launch_scan_w<24>(a, mode, block, n_cu, s);
+    case 32: return launch_scan_w<32>(a, mode, block, n_cu, s);
+    default: return launch_scan_w<0>(a, mode, block, n_cu, s);
+    }
+}
+
+hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, arr, (1u << log2_words) - 1u, steps, out);
+    return hipGetLastError();
+}
+
+}  // namespace ffbf

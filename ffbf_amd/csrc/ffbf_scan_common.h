@@ -1,0 +1,268 @@
+// ffbf_scan_common.h -- per-unit text loading and hashing shared by the plain
+// and batched scan kernels (internal).
+//
+// Work layout: the text is cut into 16-byte units aligned to 16 bytes in
+// memory; one lane owns the 16 windows that start in one unit.  A lane loads
+// its unit plus the next (W-1)/16 units with 128-bit non-temporal loads, looks
+// every byte up once in an LDS copy of the sbox and computes its 16 hash pairs
+// as one full window plus 15 rolling slides (hash_rot_sbox_2.h:34-51).
+#pragma once
+
+#include "ffbf_common.h"
+
+namespace ffbf {
+
+constexpr int kUnit = 16;
+
+// Per-wave survivor queue: capacity and the depth at which it is drained.
+// Appends add at most 64 entries, so cap >= drain_at + 63.
+template <int BLOCK>
+struct QueueGeom {
+    static constexpr int cap = 128;
+    static constexpr int drain_at = 64;
+};
+template <>
+struct QueueGeom<1024> {
+    static constexpr int cap = 120;
+    static constexpr int drain_at = 57;
+};
+
+// One aligned 16-byte text chunk c (index relative to the 16-byte aligned base
+// below the text); bytes outside [lead, lead+n) read as zero.
+static __device__ __forceinline__ uint4 load_chunk(const uint8_t *base, uint64_t lead, uint64_t n, uint64_t c)
+{
+    const uint64_t lo = c * 16u;
+    if (lo >= lead && lo + 16u <= lead + n) {
+        const u32x4 v = __builtin_nontemporal_load((const u32x4 *)(base + lo));
+        return make_uint4(v.x, v.y, v.z, v.w);
+    }
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint64_t q = lo + (uint64_t)i;
+        if (q >= lead && q < lead + n) w[i >> 2] |= (uint32_t)base[q] << (8 * (i & 3));
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Lane L receives lane (L+1) mod 64's value (DPP wavefront rotate).
+static __device__ __forceinline__ uint32_t lane_rol1(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x134 /* wave_rol:1 */, 0xF, 0xF, false);
+}
+
+// A wave owns 64 consecutive units; lane L needs units L .. L+NCH-1.  Each
+// lane loads only its own unit (lane 0 also the NCH-1 units after the wave's
+// range) and the rest arrive by rotating across the wave: one HBM read per
+// text byte instead of NCH.
+template <int NCH>
+struct WaveChunks {
+    uint4 own;
+    uint4 halo[NCH > 1 ? NCH - 1 : 1];  // meaningful on lane 0 only
+
+    __device__ __forceinline__ void load(const uint8_t *base, uint64_t lead, uint64_t n, uint64_t u0_wave)
+    {
+        own = load_chunk(base, lead, n, u0_wave + (uint64_t)lane_id());
+        if (NCH > 1 && lane_id() == 0) {
+#pragma unroll
+            for (int c = 0; c < NCH - 1; c++) halo[c] = load_chunk(base, lead, n, u0_wave + 64u + (uint64_t)c);
+        }
+    }
+
+    __device__ __forceinline__ void spread(uint4 (&ch)[NCH]) const
+    {
+        ch[0] = own;
+        const bool l0 = lane_id() == 0;
+#pragma unroll
+        for (int c = 1; c < NCH; c++) {
+            const uint4 m = l0 ? halo[c - 1] : ch[c - 1];
+            ch[c] = make_uint4(lane_rol1(m.x), lane_rol1(m.y), lane_rol1(m.z), lane_rol1(m.w));
+        }
+    }
+};
+
+template <int W>
+struct Chunks {
+    static constexpr int NB = kUnit + W - 1;
+    static constexpr int NCH = (NB + 15) / 16;
+};
+
+// Hash pairs of the 16 windows of a unit from its NCH chunks (W compile-time).
+template <int W>
+static __device__ __forceinline__ void hash_chunks(const uint4 (&ch)[Chunks<W>::NCH], const uint32_t *sb, uint32_t (&h1)[kUnit],
+                                            uint32_t (&h2)[kUnit])
+{
+    constexpr int NB = Chunks<W>::NB;
+    uint32_t s[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const uint4 &c = ch[j >> 4];
+        const int d = (j >> 2) & 3;
+        const uint32_t w = d == 0 ? c.x : d == 1 ? c.y : d == 2 ? c.z : c.w;
+        s[j] = sb[(w >> (8 * (j & 3))) & 0xFFu];
+    }
+    uint32_t x1 = 0, x2 = 0;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        x1 = rotl(x1, 1) ^ s[j];
+        x2 = rotl(x2, 3) ^ s[j];
+    }
+    h1[0] = x1;
+    h2[0] = x2;
+#pragma unroll
+    for (int b = 1; b < kUnit; b++) {
+        x1 = rotl(x1, 1) ^ s[b + W - 1] ^ rotl(s[b - 1], (uint32_t)W);
+        x2 = rotl(x2, 3) ^ s[b + W - 1] ^ rotl(s[b - 1], (uint32_t)(3 * W));
+        h1[b] = x1;
+        h2[b] = x2;
+    }
+}
+
+// Run-time W (1..64): bytes fetched one at a time.
+static __device__ __forceinline__ uint32_t byte_at(const uint8_t *base, uint64_t lead, uint64_t n, uint64_t q)
+{
+    return (q >= lead && q < lead + n) ? (uint32_t)base[q] : 0u;
+}
+
+static __device__ __forceinline__ void hash_unit_generic(const uint8_t *base, uint64_t lead, uint64_t n, uint64_t u, uint32_t W,
+                                                  const uint32_t *sb, uint32_t (&h1)[kUnit], uint32_t (&h2)[kUnit])
+{
+    const uint64_t q0 = u * 16u;
+    uint32_t x1 = 0, x2 = 0;
+    for (uint32_t j = 0; j < W; j++) {
+        const uint32_t s = sb[byte_at(base, lead, n, q0 + j)];
+        x1 = rotl(x1, 1) ^ s;
+        x2 = rotl(x2, 3) ^ s;
+    }
+    h1[0] = x1;
+    h2[0] = x2;
+#pragma unroll
+    for (int b = 1; b < kUnit; b++) {
+        const uint32_t si = sb[byte_at(base, lead, n, q0 + (uint64_t)b + W - 1)];
+        const uint32_t so = sb[byte_at(base, lead, n, q0 + (uint64_t)b - 1)];
+        x1 = rotl(x1, 1) ^ si ^ rotl(so, W);
+        x2 = rotl(x2, 3) ^ si ^ rotl(so, 3u * W);
+        h1[b] = x1;
+        h2[b] = x2;
+    }
+}
+
+// Finishes up to 64 queued windows, one per lane: the prefilter bits not yet
+// tested, then (for a fold) the k forward bits of F, then feed-forward and
+// candidate append.  Called by the whole wave.
+template <bool LDSPRE, int P>
+static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const ScanArgs &a, const uint32_t *s_pre)
+{
+    const bool act = (uint32_t)lane_id() < cnt;
+    uint4 e = make_uint4(0u, 0u, 0u, 0u);
+    if (act) e = q[lane_id()];
+    const uint32_t h1 = e.z, h2 = e.w;
+    uint32_t ok = act ? 1u : 0u;
+    if constexpr (LDSPRE) {
+        // bits P..k-1 of the LDS prefilter, loads issued back to back
+        uint32_t bits[5];
+#pragma unroll
+        for (int j = P; j < 5; j++) bits[j] = lds_bit(s_pre, fwd_idx(j, h1, h2), a.pre_log2w);
+#pragma unroll
+        for (int j = P; j < 5; j++) ok &= ((uint32_t)j < a.k) ? bits[j] : 1u;
+    } else {
+        // the global prefilter tested bit 0 only; bit 1 next, then the rest
+        // only for the few lanes still passing (each step is one round trip
+        // to L2, and L2 requests are what bounds this mode)
+        if (act && a.k > 1u) {
+            const uint32_t i1 = fwd_idx(1, h1, h2) & a.pre_mask;
+            ok &= (a.pre[i1 >> 5] >> (i1 & 31u)) & 1u;
+        }
+        if (ok && a.k > 2u) {
+            uint32_t words[5], idx[5];
+#pragma unroll
+            for (int j = 2; j < 5; j++) {
+                idx[j] = fwd_idx(j, h1, h2) & a.pre_mask;
+                words[j] = ((uint32_t)j < a.k) ? a.pre[idx[j] >> 5] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int j = 2; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
+        }
+    }
+    if (!a.pre_exact && ok) {
+        uint32_t words[5], idx[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            idx[j] = fwd_idx(j, h1, h2) & a.mask;
+            words[j] = ((uint32_t)j < a.k) ? a.F[idx[j] >> 5] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
+    }
+    const bool hit = ok != 0u;
+    const uint64_t off = ((uint64_t)e.y << 32) | e.x;
+    feed_forward(hit, h1, h2, a.k, a.mask, a.R);
+    append_cand(hit, off, h1, h2, a.cand, a.cand_cap, a.counters + C_CAND);
+}
+
+// Prefilter test of one lane's 16 windows (window b starts at text offset
+// p0+b; only offsets in [0, npos) are real) and wave-aggregated push of the
+// survivors onto the wave's LDS queue, draining it 64 at a time.  `tag` is
+// OR-ed into the recorded offsets (the batched scan stores the text id in the
+// top bits).  Called by the whole wave.
+template <bool LDSPRE, int BLOCK, int P>
+static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const uint32_t *s_pre, const uint32_t (&h1)[kUnit],
+                                                      const uint32_t (&h2)[kUnit], int64_t p0, uint64_t npos,
+                                                      uint64_t tag, uint4 *q, uint32_t &qn)
+{
+    using QG = QueueGeom<BLOCK>;
+    uint32_t vm = 0xFFFFu;
+    if (p0 < 0 || (uint64_t)p0 + kUnit > npos) {
+        vm = 0;
+#pragma unroll
+        for (int b = 0; b < kUnit; b++) {
+            const int64_t p = p0 + b;
+            vm |= (uint32_t)(p >= 0 && (uint64_t)p < npos) << b;
+        }
+    }
+    uint32_t pm = 0;
+    if constexpr (LDSPRE) {
+        // P bits per window, all loads in flight together
+        uint32_t bits[P][kUnit];
+#pragma unroll
+        for (int j = 0; j < P; j++)
+#pragma unroll
+            for (int b = 0; b < kUnit; b++) bits[j][b] = lds_bit(s_pre, fwd_idx(j, h1[b], h2[b]), a.pre_log2w);
+#pragma unroll
+        for (int b = 0; b < kUnit; b++) {
+            uint32_t ok = bits[0][b];
+#pragma unroll
+            for (int j = 1; j < P; j++) ok &= bits[j][b];
+            pm |= ok << b;
+        }
+    } else {
+        uint32_t fw[kUnit];
+#pragma unroll
+        for (int b = 0; b < kUnit; b++) fw[b] = a.pre[(h1[b] & a.pre_mask) >> 5];
+#pragma unroll
+        for (int b = 0; b < kUnit; b++) pm |= ((fw[b] >> (h1[b] & 31u)) & 1u) << b;
+    }
+    pm &= vm;
+    // one drain site: b is wave-uniform, so h1[b]/h2[b] index registers
+    // directly (no scratch) and the loop stays rolled
+#pragma unroll 1
+    for (uint32_t b = 0; b < (uint32_t)kUnit; b++) {
+        const bool pass = (pm >> b) & 1u;
+        const uint64_t bal = __ballot(pass);
+        if (!bal) continue;
+        if (pass) {
+            const uint64_t p = (uint64_t)(p0 + (int64_t)b) | tag;
+            q[qn + lanes_below(bal)] = make_uint4((uint32_t)p, (uint32_t)(p >> 32), h1[b], h2[b]);
+        }
+        qn += (uint32_t)__builtin_popcountll(bal);
+        if (qn >= (uint32_t)QG::drain_at) {
+            const uint32_t cnt = qn < 64u ? qn : 64u;
+            wave_sync();
+            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre);
+            wave_sync();
+            qn -= cnt;
+        }
+    }
+}
+
+}  // namespace ffbf

@@ -75,8 +75,10 @@ typedef struct ffbf_timings {
     float scan_kernel_ms;  /* the scan kernel alone */
     float pattern_pass_ms; /* pattern pass (+ survivor index) */
     float verify_ms;       /* exact verification */
-    uint32_t scan_mode;    /* 0 global first probe, 1 LDS fold prefilter, 2 whole filter in LDS */
-    uint32_t fold_log2_bits;
+    uint32_t scan_mode;      /* 0 F probed in HBM/L2, 1 LDS fold, 2 whole F in LDS, 3 L2-resident fold */
+    uint32_t fold_log2_bits; /* prefilter size (0 in mode 0) */
+    uint32_t probes;         /* forward bits tested against the prefilter before queueing */
+    uint32_t block;          /* scan workgroup size */
 } ffbf_timings;
 
 typedef struct ffbf_ctx ffbf_ctx;
@@ -149,8 +151,18 @@ int ffbf_filter_load(ffbf_ctx *ctx, const uint32_t *deltas, uint64_t n);
 /* How the scan tests windows before F in HBM (results never depend on it):
  * mode -1 picks automatically after each build; 0 first probe in F;
  * 1 OR-fold of F held in LDS, fold_log2_bits bits (0 = largest that fits);
- * 2 whole filter in LDS (only when it fits).  Takes effect at the next scan. */
+ * 2 whole filter in LDS (only when it fits); 3 OR-fold of F in global memory
+ * sized to stay in L2 (fold_log2_bits, 0 = 2^24).  A fold bit is the OR of
+ * all F bits congruent to it, so the fold never rejects a true candidate.
+ * Takes effect at the next scan. */
 int ffbf_set_scan_mode(ffbf_ctx *ctx, int mode, uint32_t fold_log2_bits);
+
+/* Launch tuning of the scan (results never depend on it): probes = forward
+ * bits tested against an LDS prefilter before a window is queued (0 auto,
+ * else 1..k; more than 3 acts as 3); block = workgroup size of the LDS
+ * prefilter modes (0 auto = 1024, or 512); the global modes always use 256.
+ * ffbf_get_timings reports the effective values. */
+int ffbf_set_scan_tuning(ffbf_ctx *ctx, uint32_t probes, uint32_t block);
 
 int ffbf_get_timings(ffbf_ctx *ctx, ffbf_timings *out);
 int ffbf_synchronize(ffbf_ctx *ctx);
@@ -165,6 +177,14 @@ int ffbf_device_free(void *ptr);
 int ffbf_device_upload(void *dst, const void *src, uint64_t bytes);
 int ffbf_device_download(void *dst, const void *src, uint64_t bytes);
 int ffbf_device_count(int *out);
+/* Plain copy kernel over 16-byte aligned device buffers (synchronous); the
+ * reference point for HBM byte counters. */
+int ffbf_device_copy(void *dst, const void *src, uint64_t bytes);
+
+/* Random-gather microbenchmark: 16 independent uint32 loads per lane per
+ * step at hashed addresses of a 2^log2_bytes-byte device array, over the
+ * whole device.  Reports the time of one launch and the gather rate. */
+int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms, double *probes_per_s);
 
 #ifdef __cplusplus
 }

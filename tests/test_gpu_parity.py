@@ -50,7 +50,7 @@ def _case_arrays(case):
     return pats, b, off, text
 
 
-@pytest.mark.parametrize("mode", [fa.MODE_GLOBAL, fa.MODE_FOLD, fa.MODE_LDS, fa.MODE_AUTO])
+@pytest.mark.parametrize("mode", [fa.MODE_GLOBAL, fa.MODE_FOLD, fa.MODE_LDS, fa.MODE_GFOLD, fa.MODE_AUTO])
 @pytest.mark.parametrize("ci", [0, 1, 2])
 def test_golden_cases(ref_sbox, ci, mode):
     case = load_golden("e2e_small.json")["cases"][ci]
@@ -59,7 +59,8 @@ def test_golden_cases(ref_sbox, ci, mode):
     with fa.Matcher(case["window"], case["k"], case["log2_bits"], ref_sbox) as m:
         m.set_patterns(b, off)
         m.build()
-        m.set_scan_mode(mode, case["log2_bits"] - 2 if mode == fa.MODE_FOLD else 0)
+        fold = {fa.MODE_FOLD: case["log2_bits"] - 2, fa.MODE_GFOLD: max(10, case["log2_bits"] - 2)}.get(mode, 0)
+        m.set_scan_mode(mode, fold)
         assert bits_of(m.filter(fa.FILTER_FORWARD)) == case["forward_bits"]
         g = gpu_all(m, text)
         assert g["cand"].tolist() == case["candidates"]
@@ -91,6 +92,9 @@ RANDOM_CASES = [
     (fa.ALPHA_DNA_GC40, 64, 1, 16, 1000, 60, 90, 400_000, 100, fa.MODE_GLOBAL),  # W=64, k=1
     (fa.ALPHA_ASCII, 1, 4, 10, 20, 1, 3, 50_000, 10, fa.MODE_AUTO),            # W=1
     (fa.ALPHA_DNA_GC40, 19, 5, 26, 5000, 19, 64, 2_000_000, 500, fa.MODE_AUTO),
+    (fa.ALPHA_DNA_GC40, 32, 4, 22, 20000, 32, 100, 3_000_000, 500, fa.MODE_GFOLD),
+    (fa.ALPHA_ASCII, 24, 3, 20, 5000, 24, 80, 1_000_000, 300, fa.MODE_GFOLD),
+    (fa.ALPHA_ASCII, 13, 4, 18, 5000, 13, 40, 1_000_000, 300, fa.MODE_GFOLD),     # generic W
 ]
 
 
@@ -107,6 +111,8 @@ def test_random_vs_oracle(ci):
         m.build()
         if mode == fa.MODE_FOLD:
             m.set_scan_mode(mode, max(7, log2 - 3))
+        elif mode == fa.MODE_GFOLD:
+            m.set_scan_mode(mode, max(10, log2 - 4))
         else:
             m.set_scan_mode(mode)
         g = gpu_all(m, text)
@@ -223,19 +229,37 @@ def test_filter_persistence_round_trip():
         assert_same(gpu_all(m2, text), r, m2.filter())
 
 
-def test_scan_modes_agree_on_one_input():
-    say("mode agreement")
-    sbox = fa.sbox_from_seed(10)
-    b, off, text = make_case(fa.ALPHA_ASCII, 19, 4000, 20, 64, 3_000_000, 300, 41)
-    outs = []
-    with fa.Matcher(19, 4, 20, sbox) as m:
+SCAN_CONFIGS = [
+    # mode, fold log2 bits, probes, block -> effective block
+    (fa.MODE_GLOBAL, 0, 0, 0, 256), (fa.MODE_GLOBAL, 0, 0, 512, 256),
+    (fa.MODE_FOLD, 14, 0, 0, 1024), (fa.MODE_FOLD, 17, 1, 512, 512), (fa.MODE_FOLD, 17, 2, 1024, 1024),
+    (fa.MODE_FOLD, 20, 4, 512, 512), (fa.MODE_FOLD, 18, 3, 0, 1024),
+    (fa.MODE_LDS, 0, 1, 512, 512), (fa.MODE_LDS, 0, 3, 1024, 1024), (fa.MODE_LDS, 0, 4, 0, 1024),
+    (fa.MODE_LDS, 0, 2, 0, 1024),
+    (fa.MODE_GFOLD, 12, 0, 0, 256), (fa.MODE_GFOLD, 16, 0, 512, 256), (fa.MODE_GFOLD, 20, 0, 0, 256),
+]
+
+
+@pytest.mark.parametrize("W", [19, 24, 32, 21])
+def test_scan_configs_agree_on_one_input(W):
+    say(f"config agreement W={W}")
+    sbox = fa.sbox_from_seed(10 + W)
+    b, off, text = make_case(fa.ALPHA_ASCII, W, 4000, W, 64, 3_000_011, 300, 41)
+    o = Oracle(W, 4, 20, sbox)
+    r = o.run(text, b, off, threads=8)
+    with fa.Matcher(W, 4, 20, sbox) as m:
         m.set_patterns(b, off)
         m.build()
-        for mode, fold in ((fa.MODE_GLOBAL, 0), (fa.MODE_FOLD, 14), (fa.MODE_FOLD, 17), (fa.MODE_LDS, 0)):
+        for mode, fold, probes, block, eff in SCAN_CONFIGS:
+            say(f"  mode={mode} fold={fold} probes={probes} block={block}")
             m.set_scan_mode(mode, fold)
-            outs.append(gpu_all(m, text))
-            assert m.timings()["scan_mode"] == mode
-    for g in outs[1:]:
-        for key in ("cand", "R", "surv"):
-            assert np.array_equal(g[key], outs[0][key])
-        assert np.array_equal(g["matches"], outs[0]["matches"])
+            m.set_scan_tuning(probes, block)
+            assert_same(gpu_all(m, text), r)
+            t = m.timings()
+            assert t["scan_mode"] == mode
+            assert t["block"] == eff
+
+
+def test_gather_bench_runs():
+    ms, rate = fa.gather_bench(20, steps=4)
+    assert ms > 0 and rate > 1e9
