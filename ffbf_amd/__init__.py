@@ -81,6 +81,19 @@ class Timings(ctypes.Structure):
 
 
 MATCH_DTYPE = np.dtype([("pattern_id", "<u4"), ("reserved", "<u4"), ("offset", "<u8")])
+BATCH_MATCH_DTYPE = np.dtype([("text_id", "<u4"), ("pattern_id", "<u4"), ("offset", "<u8")])
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("chunk_iters", ctypes.c_uint32),
+        ("probes", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("use_graph", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 _c = ctypes
 _P = _c.c_void_p
@@ -121,6 +134,16 @@ ABI = {
     "ffbf_device_download": (_i32, [_P, _P, _u64]),
     "ffbf_device_count": (_i32, [_c.POINTER(_i32)]),
     "ffbf_device_copy": (_i32, [_P, _P, _u64]),
+    "ffbf_batch_create": (_i32, [_P, _u32, _c.POINTER(_P)]),
+    "ffbf_batch_destroy": (None, [_P]),
+    "ffbf_batch_set_texts": (_i32, [_P, _P, _P, _P, _u32]),
+    "ffbf_batch_run": (_i32, [_P, _pu64]),
+    "ffbf_batch_run_per_text": (_i32, [_P, _pu64]),
+    "ffbf_batch_get_matches": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_batch_set_tuning": (_i32, [_P, _c.POINTER(BatchTuning)]),
+    "ffbf_batch_get_tuning": (_i32, [_P, _c.POINTER(BatchTuning)]),
+    "ffbf_batch_autotune": (_i32, [_P, _c.POINTER(BatchTuning), _c.POINTER(_c.c_float)]),
+    "ffbf_batch_timings": (_i32, [_P, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float)]),
     # workload generators (host only)
     "ffbf_gen_bytes": (_i32, [_u32, _u64, _u64, _u64, _P, _i32]),
     "ffbf_gen_pattern_offsets": (_i32, [_u64, _u64, _u32, _u32, _P]),
@@ -430,3 +453,79 @@ class Matcher:
     @property
     def stream(self) -> int:
         return _lib.ffbf_get_stream(self._h) or 0
+
+
+class Batch:
+    """Many device texts, each routed to one of several resident Matchers,
+    scanned in one persistent launch (replayed from a captured hipGraph)."""
+
+    def __init__(self, matchers: Sequence[Matcher]):
+        self._matchers = list(matchers)  # keep the contexts alive
+        arr = (_c.c_void_p * len(self._matchers))(*[m._h.value for m in self._matchers])
+        h = _c.c_void_p(0)
+        _check(_lib.ffbf_batch_create(arr, len(self._matchers), _c.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.ffbf_batch_destroy(self._h)
+            self._h = _c.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_texts(self, dev_ptr: int, offsets: np.ndarray, filters: np.ndarray) -> None:
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        fil = np.ascontiguousarray(filters, dtype=np.uint32)
+        if off.size != fil.size + 1:
+            raise ValueError("offsets needs one more entry than filters")
+        _check(_lib.ffbf_batch_set_texts(self._h, dev_ptr, _ptr(off), _ptr(fil), fil.size))
+
+    def run(self) -> int:
+        n = _c.c_uint64(0)
+        _check(_lib.ffbf_batch_run(self._h, _c.byref(n)))
+        return n.value
+
+    def run_per_text(self) -> int:
+        n = _c.c_uint64(0)
+        _check(_lib.ffbf_batch_run_per_text(self._h, _c.byref(n)))
+        return n.value
+
+    def matches(self) -> np.ndarray:
+        n = _c.c_uint64(0)
+        st = _lib.ffbf_batch_get_matches(self._h, None, 0, _c.byref(n))
+        if st not in (OK, ECAPACITY):
+            _check(st)
+        out = np.zeros(n.value, dtype=BATCH_MATCH_DTYPE)
+        _check(_lib.ffbf_batch_get_matches(self._h, _ptr(out), n.value, _c.byref(n)))
+        return out
+
+    def set_tuning(self, chunk_iters: int = 0, probes: int = 0, block: int = 0, use_graph: bool = True) -> None:
+        t = BatchTuning(chunk_iters, probes, block, 1 if use_graph else 0)
+        _check(_lib.ffbf_batch_set_tuning(self._h, _c.byref(t)))
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        _check(_lib.ffbf_batch_get_tuning(self._h, _c.byref(t)))
+        return t.as_dict()
+
+    def autotune(self):
+        t = BatchTuning()
+        ms = _c.c_float(0)
+        _check(_lib.ffbf_batch_autotune(self._h, _c.byref(t), _c.byref(ms)))
+        return t.as_dict(), ms.value
+
+    def timings(self):
+        run = _c.c_float(0)
+        scan = _c.c_float(0)
+        _check(_lib.ffbf_batch_timings(self._h, _c.byref(run), _c.byref(scan)))
+        return run.value, scan.value

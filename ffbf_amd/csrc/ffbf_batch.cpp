@@ -1,0 +1,558 @@
+// ffbf_batch.cpp -- batched scans: many texts, each routed to one of several
+// resident filters (contexts), scanned in one persistent launch.
+//
+// One batch object binds a fixed set of built contexts (same device, window,
+// k and sbox).  A run clears every filter's R and survivor index, runs the
+// batched scan over all chunks, the pattern pass of every filter and the
+// batched verification, and copies the counters back.  That sequence does not
+// depend on the texts (chunk count and descriptors live in device memory), so
+// it is captured once in a hipGraph and replayed for every batch until a
+// buffer has to grow.  R and the survivors are per filter and per batch: a
+// filter's R is the union over the texts routed to it, which only widens the
+// survivor set; verification is per text, so every text's match list is the
+// same as when it is scanned alone.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ffbf_ctx.h"
+
+using namespace ffbf;
+
+struct ffbf_batch {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    std::vector<ffbf_ctx *> filters;
+    uint32_t W = 0, k = 0;
+    uint32_t pre_words_max = 0;  // largest LDS prefilter among the filters (dynamic LDS of the scan)
+
+    DevBuf<ScanArgs> d_scan;
+    DevBuf<VerifyArgs> d_ver;
+    DevBuf<BatchText> d_texts;
+    DevBuf<BatchChunk> d_chunks;
+    DevBuf<unsigned long long> ctl;       // [0] next chunk, [1] chunks
+    DevBuf<unsigned long long> counters;  // C_CAND, C_MATCH
+    unsigned long long *h_counters = nullptr;
+    DevBuf<Cand> cand;
+    DevBuf<BatchMatchRec> matches;
+    DevBuf<uint64_t> sort_keys;
+    DevBuf<uint8_t> sort_temp;
+
+    const uint8_t *texts = nullptr;
+    uint32_t n_texts = 0, n_chunks = 0;
+    uint64_t total_bytes = 0, max_text = 0;
+    std::vector<BatchText> h_texts;
+    bool have_texts = false;
+
+    ffbf_batch_tuning tuning{32u, 0u, 1024u, 1u};
+    uint32_t probes_eff = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e_run0 = nullptr, e_run1 = nullptr, e_scan0 = nullptr, e_scan1 = nullptr;
+    float run_ms = 0.0f, scan_ms = -1.0f;
+
+    uint64_t n_cand = 0, n_match = 0;
+    bool ran = false, sorted_on_device = false, sorted_valid = false;
+    std::vector<BatchMatchRec> sorted;
+};
+
+namespace {
+
+void drop_graph(ffbf_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+uint32_t bits_for(uint64_t v)
+{
+    uint32_t b = 1;
+    while (b < 64 && (1ull << b) <= v) b++;
+    return b;
+}
+
+// Rebuilds the per-filter descriptors from the contexts' current state.
+int refresh_filters(ffbf_batch *b)
+{
+    const uint32_t nf = (uint32_t)b->filters.size();
+    std::vector<ScanArgs> sa(nf);
+    std::vector<VerifyArgs> va(nf);
+    b->pre_words_max = 0;
+    uint32_t kmin_lds = 5;
+    for (uint32_t f = 0; f < nf; f++) {
+        const ffbf_ctx *c = b->filters[f];
+        const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
+        if (lds) {
+            b->pre_words_max = std::max(b->pre_words_max, c->pre_words);
+            kmin_lds = std::min(kmin_lds, c->params.k);
+        }
+    }
+    // probes: requested, else the smallest auto choice among LDS filters
+    uint32_t p = b->tuning.probes;
+    if (!p) {
+        p = 3;
+        for (uint32_t f = 0; f < nf; f++) {
+            const ffbf_ctx *c = b->filters[f];
+            if (c->mode == MODE_FOLD || c->mode == MODE_LDS) p = std::min(p, std::max(1u, c->probes));
+        }
+    }
+    b->probes_eff = std::max(1u, std::min({p, 3u, b->k}));
+    for (uint32_t f = 0; f < nf; f++) {
+        const ffbf_ctx *c = b->filters[f];
+        ScanArgs &a = sa[f];
+        a = ScanArgs{};
+        const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
+        a.F = c->F.p;
+        a.R = c->R.p;
+        a.pre = c->mode == MODE_FOLD ? c->fold.p : c->mode == MODE_GFOLD ? c->gfold.p : c->F.p;
+        a.sbox = c->sbox.p;
+        a.cand = b->cand.p;
+        a.cand_cap = b->cand.cap;
+        a.counters = b->counters.p;
+        a.W = c->params.window;
+        a.k = c->params.k;
+        a.mask = c->mask;
+        a.pre_words = c->pre_words;
+        a.pre_mask = c->mode == MODE_GLOBAL ? c->mask : (uint32_t)((uint64_t)c->pre_words * 32u - 1u);
+        a.pre_log2w = pow2_at_least(c->pre_words);
+        a.P = lds ? b->probes_eff : 1u;
+        a.pre_exact = (c->mode == MODE_GLOBAL || c->mode == MODE_LDS) ? 1u : 0u;
+        a.lds = lds ? 1u : 0u;
+        VerifyArgs &v = va[f];
+        v = VerifyArgs{};
+        v.bytes = c->bytes.p;
+        v.off = c->off.p;
+        v.pat_hash = c->pat_hash.p;
+        v.table = c->table.p;
+        v.slot_chain = c->table.p + c->table_size;
+        v.chain_next = c->chain_next.p;
+        v.table_mask = c->table_size - 1u;
+    }
+    RET_IF(b->d_scan.ensure(nf, "batch filter descriptors"));
+    RET_IF(b->d_ver.ensure(nf, "batch verify descriptors"));
+    HIP_TRY(hipMemcpyAsync(b->d_scan.p, sa.data(), nf * sizeof(ScanArgs), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_ver.p, va.data(), nf * sizeof(VerifyArgs), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return FFBF_OK;
+}
+
+// The whole batch run on b->stream (capturable: kernels and one D2H copy).
+int enqueue_run(ffbf_batch *b, bool timed)
+{
+    HIP_TRY(launch_fill32((uint32_t *)b->ctl.p, 2u, 0u, b->stream));  // next chunk = 0
+    HIP_TRY(launch_fill32((uint32_t *)b->counters.p, 2u * C_COUNT, 0u, b->stream));
+    for (ffbf_ctx *c : b->filters) {
+        HIP_TRY(launch_fill32(c->R.p, c->f_words, 0u, b->stream));
+        HIP_TRY(launch_fill32(c->table.p, 2ull * c->table_size, 0u, b->stream));
+        HIP_TRY(launch_fill32((uint32_t *)(c->counters.p + C_SURV), 2u, 0u, b->stream));
+    }
+    if (timed) HIP_TRY(hipEventRecord(b->e_scan0, b->stream));
+    BatchArgs ba{};
+    ba.texts = b->texts;
+    ba.tdesc = b->d_texts.p;
+    ba.chunks = b->d_chunks.p;
+    ba.filters = b->d_scan.p;
+    ba.sbox = b->filters[0]->sbox.p;
+    ba.ctl = b->ctl.p;
+    ba.W = b->W;
+    HIP_TRY(launch_scan_batch(ba, b->tuning.block, b->probes_eff, b->pre_words_max, b->n_cu, b->stream));
+    if (timed) HIP_TRY(hipEventRecord(b->e_scan1, b->stream));
+    for (ffbf_ctx *c : b->filters) {
+        PassArgs pa{};
+        pa.pat_hash = c->pat_hash.p;
+        pa.off = c->off.p;
+        pa.n = c->n_pat;
+        pa.W = c->params.window;
+        pa.k = c->params.k;
+        pa.mask = c->mask;
+        pa.R = c->R.p;
+        pa.bitmap = c->bitmap.p;
+        pa.surv_ids = c->surv_ids.p;
+        pa.counters = c->counters.p;
+        pa.table = c->table.p;
+        pa.slot_chain = c->table.p + c->table_size;
+        pa.chain_next = c->chain_next.p;
+        pa.table_mask = c->table_size - 1u;
+        HIP_TRY(launch_pattern_pass(pa, c->n_cu, b->stream));
+    }
+    BatchVerifyArgs va{};
+    va.texts = b->texts;
+    va.tdesc = b->d_texts.p;
+    va.cand = b->cand.p;
+    va.cand_cap = b->cand.cap;
+    va.filters = b->d_ver.p;
+    va.out = b->matches.p;
+    va.out_cap = b->matches.cap;
+    va.counters = b->counters.p;
+    HIP_TRY(launch_verify_batch(va, b->n_cu, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->h_counters, b->counters.p, C_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           b->stream));
+    return FFBF_OK;
+}
+
+int run_once(ffbf_batch *b)
+{
+    const bool graph = b->tuning.use_graph != 0u;
+    if (graph && !b->exec) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int st = enqueue_run(b, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st != FFBF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIP_TRY(e);
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipEventRecord(b->e_run0, b->stream));
+    if (graph) {
+        HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        RET_IF(enqueue_run(b, true));
+    }
+    HIP_TRY(hipEventRecord(b->e_run1, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    (void)hipEventElapsedTime(&b->run_ms, b->e_run0, b->e_run1);
+    b->scan_ms = -1.0f;
+    if (!graph) (void)hipEventElapsedTime(&b->scan_ms, b->e_scan0, b->e_scan1);
+    return FFBF_OK;
+}
+
+int sort_on_device(ffbf_batch *b)
+{
+    b->sorted_on_device = false;
+    if (b->n_match < 2) return FFBF_OK;
+    uint64_t max_pat = 1;
+    for (const ffbf_ctx *c : b->filters) max_pat = std::max<uint64_t>(max_pat, c->n_pat);
+    const uint32_t bo = bits_for(b->max_text), bi = bits_for(max_pat), bt = bits_for(b->n_texts);
+    if (bo + bi + bt > 64) return FFBF_OK;
+    const int end_bit = (int)(bo + bi + bt);
+    size_t tb = 0;
+    HIP_TRY(sort_temp_bytes(b->n_match, end_bit, &tb));
+    RET_IF(b->sort_keys.ensure(2 * b->n_match, "batch sort keys"));
+    RET_IF(b->sort_temp.ensure(tb, "batch sort scratch"));
+    HIP_TRY(sort_batch_matches(b->matches.p, b->n_match, b->sort_keys.p, b->sort_keys.p + b->n_match, b->sort_temp.p, tb,
+                               (int)bo, (int)(bo + bi), end_bit, b->stream));
+    b->sorted_on_device = true;
+    return FFBF_OK;
+}
+
+int check_batch(ffbf_batch *b)
+{
+    if (!b) return fail(FFBF_EINVAL, "null batch");
+    HIP_TRY(hipSetDevice(b->device));
+    return FFBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch **out)
+{
+    if (!filters || !out || n_filters == 0) return fail(FFBF_EINVAL, "need at least one filter");
+    *out = nullptr;
+    const ffbf_ctx *f0 = filters[0];
+    if (!f0) return fail(FFBF_EINVAL, "null filter");
+    for (uint32_t i = 0; i < n_filters; i++) {
+        const ffbf_ctx *c = filters[i];
+        if (!c) return fail(FFBF_EINVAL, "null filter");
+        if (!c->built || !c->patterns) return fail(FFBF_ESTATE, "every filter must have patterns and a built filter");
+        if (c->device != f0->device || c->params.window != f0->params.window || c->params.k != f0->params.k ||
+            std::memcmp(c->params.sbox, f0->params.sbox, sizeof f0->params.sbox) != 0)
+            return fail(FFBF_EINVAL, "filters of a batch need the same device, window, k and sbox");
+    }
+    ffbf_batch *b = new (std::nothrow) ffbf_batch();
+    if (!b) return fail(FFBF_ENOMEM, "batch allocation");
+    b->filters.assign(filters, filters + n_filters);
+    b->device = f0->device;
+    b->W = f0->params.window;
+    b->k = f0->params.k;
+    auto init = [&]() -> int {
+        HIP_TRY(hipSetDevice(b->device));
+        HIP_TRY(hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+        HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&b->e_run0));
+        HIP_TRY(hipEventCreate(&b->e_run1));
+        HIP_TRY(hipEventCreate(&b->e_scan0));
+        HIP_TRY(hipEventCreate(&b->e_scan1));
+        HIP_TRY(hipHostMalloc((void **)&b->h_counters, C_COUNT * sizeof(unsigned long long), hipHostMallocDefault));
+        RET_IF(b->ctl.ensure(2, "batch control"));
+        RET_IF(b->counters.ensure(C_COUNT, "batch counters"));
+        RET_IF(b->cand.ensure(1 << 16, "batch candidates"));
+        RET_IF(b->matches.ensure(1 << 16, "batch matches"));
+        // the filters' own streams may still be building
+        for (ffbf_ctx *c : b->filters) HIP_TRY(hipStreamSynchronize(c->stream));
+        return refresh_filters(b);
+    };
+    const int st = init();
+    if (st != FFBF_OK) {
+        const std::string msg = ffbf_last_error();
+        ffbf_batch_destroy(b);
+        fail(st, msg);
+        return st;
+    }
+    *out = b;
+    return FFBF_OK;
+}
+
+void ffbf_batch_destroy(ffbf_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    b->d_scan.release();
+    b->d_ver.release();
+    b->d_texts.release();
+    b->d_chunks.release();
+    b->ctl.release();
+    b->counters.release();
+    b->cand.release();
+    b->matches.release();
+    b->sort_keys.release();
+    b->sort_temp.release();
+    if (b->h_counters) (void)hipHostFree(b->h_counters);
+    for (hipEvent_t e : {b->e_run0, b->e_run1, b->e_scan0, b->e_scan1})
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    (void)hipGetLastError();
+    delete b;
+}
+
+int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t *offsets, const uint32_t *filter_of_text,
+                         uint32_t n_texts)
+{
+    RET_IF(check_batch(b));
+    if (!offsets || (n_texts && (!filter_of_text || !dev_texts))) return fail(FFBF_EINVAL, "null argument");
+    if (n_texts >= (1u << (64 - kBatchOffsetBits))) return fail(FFBF_EINVAL, "too many texts in one batch");
+    const uint32_t nf = (uint32_t)b->filters.size();
+    std::vector<BatchText> tx(n_texts);
+    uint64_t total = 0, maxn = 0;
+    for (uint32_t i = 0; i < n_texts; i++) {
+        if (offsets[i + 1] < offsets[i]) return fail(FFBF_EINVAL, "offsets must be non-decreasing");
+        if (filter_of_text[i] >= nf) return fail(FFBF_EINVAL, "filter id out of range");
+        tx[i].start = offsets[i];
+        tx[i].n = offsets[i + 1] - offsets[i];
+        tx[i].filter = filter_of_text[i];
+        tx[i].reserved = 0;
+        if (tx[i].n >= (1ull << kBatchOffsetBits)) return fail(FFBF_EINVAL, "text longer than 2^40 bytes");
+        total += tx[i].n;
+        maxn = std::max(maxn, tx[i].n);
+    }
+    // chunks of chunk_iters 1 KiB iterations, ordered by filter then text
+    std::vector<uint32_t> order(n_texts);
+    for (uint32_t i = 0; i < n_texts; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return tx[x].filter < tx[y].filter; });
+    const uint32_t ci = b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u;
+    std::vector<BatchChunk> chunks;
+    chunks.reserve(total / (1024ull * ci) + n_texts + 1);
+    for (uint32_t i : order) {
+        const uint64_t lead = (uint64_t)((uintptr_t)(dev_texts + tx[i].start) & 15u);
+        const uint64_t npos = tx[i].n >= b->W ? tx[i].n - b->W + 1 : 0;
+        if (!npos) continue;
+        const uint64_t units = (lead + npos + 15u) >> 4;
+        const uint64_t iters = (units + 63u) >> 6;
+        if (iters > 0xFFFFFFFFull) return fail(FFBF_EINVAL, "text too long for one batch");
+        for (uint64_t it = 0; it < iters; it += ci) {
+            BatchChunk c;
+            c.text = i;
+            c.filter = tx[i].filter;
+            c.it0 = (uint32_t)it;
+            c.n_it = (uint32_t)std::min<uint64_t>(ci, iters - it);
+            chunks.push_back(c);
+        }
+    }
+    const bool grow = tx.size() > b->d_texts.cap || chunks.size() > b->d_chunks.cap;
+    if (grow) drop_graph(b);
+    RET_IF(b->d_texts.ensure(std::max<uint64_t>(tx.size() + tx.size() / 4, 64), "batch text descriptors"));
+    RET_IF(b->d_chunks.ensure(std::max<uint64_t>(chunks.size() + chunks.size() / 4, 256), "batch chunks"));
+    if (b->texts != dev_texts) drop_graph(b);
+    b->texts = dev_texts;
+    b->n_texts = n_texts;
+    b->n_chunks = (uint32_t)chunks.size();
+    b->total_bytes = total;
+    b->max_text = maxn;
+    b->h_texts = tx;
+    const uint64_t want = std::max<uint64_t>(1 << 16, total >> 10);
+    if (b->cand.cap < want) {
+        drop_graph(b);
+        RET_IF(b->cand.ensure(want, "batch candidates"));
+        RET_IF(refresh_filters(b));
+    }
+    if (n_texts) HIP_TRY(hipMemcpyAsync(b->d_texts.p, tx.data(), tx.size() * sizeof(BatchText), hipMemcpyHostToDevice, b->stream));
+    if (!chunks.empty())
+        HIP_TRY(hipMemcpyAsync(b->d_chunks.p, chunks.data(), chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice,
+                               b->stream));
+    const unsigned long long ctl[2] = {0ull, (unsigned long long)chunks.size()};
+    HIP_TRY(hipMemcpyAsync(b->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->have_texts = true;
+    b->ran = false;
+    return FFBF_OK;
+}
+
+int ffbf_batch_run(ffbf_batch *b, uint64_t *n_matches)
+{
+    RET_IF(check_batch(b));
+    if (!b->have_texts) return fail(FFBF_ESTATE, "no texts set");
+    for (int attempt = 0; attempt < 3; attempt++) {
+        RET_IF(run_once(b));
+        const uint64_t nc = b->h_counters[C_CAND], nm = b->h_counters[C_MATCH];
+        if (nc <= b->cand.cap && nm <= b->matches.cap) break;
+        // grow and run again: every stage is a pure function of its inputs
+        drop_graph(b);
+        if (nc > b->cand.cap) RET_IF(b->cand.ensure(nc + nc / 4 + 1024, "batch candidates"));
+        if (nm > b->matches.cap) RET_IF(b->matches.ensure(nm + nm / 4 + 1024, "batch matches"));
+        RET_IF(refresh_filters(b));
+        if (attempt == 2) return fail(FFBF_EDEVICE, "batch outputs kept overflowing");
+    }
+    b->n_cand = b->h_counters[C_CAND];
+    b->n_match = b->h_counters[C_MATCH];
+    b->ran = true;
+    b->sorted_valid = false;
+    RET_IF(sort_on_device(b));
+    if (n_matches) *n_matches = b->n_match;
+    return FFBF_OK;
+}
+
+int ffbf_batch_get_matches(ffbf_batch *b, ffbf_batch_match *out, uint64_t cap, uint64_t *n)
+{
+    RET_IF(check_batch(b));
+    if (!b->ran) return fail(FFBF_ESTATE, "batch has not run");
+    if (n) *n = b->n_match;
+    if (cap < b->n_match) return fail(FFBF_ECAPACITY, "match buffer too small");
+    if (b->n_match && !out) return fail(FFBF_EINVAL, "null output");
+    if (!b->sorted_valid) {
+        b->sorted.resize(b->n_match);
+        if (b->n_match) {
+            HIP_TRY(hipMemcpyAsync(b->sorted.data(), b->matches.p, b->n_match * sizeof(BatchMatchRec),
+                                   hipMemcpyDeviceToHost, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+        }
+        if (!b->sorted_on_device)
+            std::sort(b->sorted.begin(), b->sorted.end(), [](const BatchMatchRec &x, const BatchMatchRec &y) {
+                if (x.text != y.text) return x.text < y.text;
+                return x.id != y.id ? x.id < y.id : x.offset < y.offset;
+            });
+        b->sorted_valid = true;
+    }
+    for (uint64_t i = 0; i < b->n_match; i++) {
+        out[i].text_id = b->sorted[i].text;
+        out[i].pattern_id = b->sorted[i].id;
+        out[i].offset = b->sorted[i].offset;
+    }
+    return FFBF_OK;
+}
+
+int ffbf_batch_run_per_text(ffbf_batch *b, uint64_t *n_matches)
+{
+    RET_IF(check_batch(b));
+    if (!b->have_texts) return fail(FFBF_ESTATE, "no texts set");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<BatchMatchRec> all;
+    std::vector<ffbf_match> one;
+    for (uint32_t i = 0; i < b->n_texts; i++) {
+        ffbf_ctx *c = b->filters[b->h_texts[i].filter];
+        uint64_t nm = 0;
+        RET_IF(ffbf_run_device(c, b->texts + b->h_texts[i].start, b->h_texts[i].n, &nm));
+        one.resize(nm);
+        RET_IF(ffbf_get_matches(c, one.data(), nm, &nm));
+        for (const ffbf_match &m : one) all.push_back(BatchMatchRec{i, m.pattern_id, m.offset});
+    }
+    // each text ran to completion (with its own synchronisation) inside the loop
+    b->run_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    b->scan_ms = -1.0f;
+    b->sorted = std::move(all);
+    b->n_match = b->sorted.size();
+    b->sorted_valid = true;
+    b->ran = true;
+    if (n_matches) *n_matches = b->n_match;
+    return FFBF_OK;
+}
+
+int ffbf_batch_set_tuning(ffbf_batch *b, const ffbf_batch_tuning *t)
+{
+    RET_IF(check_batch(b));
+    if (!t) return fail(FFBF_EINVAL, "null tuning");
+    if (t->block != 0 && t->block != 512 && t->block != 1024) return fail(FFBF_EINVAL, "block must be 0, 512 or 1024");
+    if (t->probes > b->k) return fail(FFBF_EINVAL, "probes must be 0..k");
+    if (t->chunk_iters > (1u << 20)) return fail(FFBF_EINVAL, "chunk_iters too large");
+    const bool rechunk = (t->chunk_iters ? t->chunk_iters : 32u) != (b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u);
+    b->tuning = *t;
+    if (!b->tuning.block) b->tuning.block = 1024;
+    if (!b->tuning.chunk_iters) b->tuning.chunk_iters = 32;
+    drop_graph(b);
+    RET_IF(refresh_filters(b));
+    if (rechunk && b->have_texts) {
+        // re-cut the current texts with the new chunk size
+        std::vector<uint64_t> off(b->n_texts + 1);
+        std::vector<uint32_t> fil(b->n_texts);
+        for (uint32_t i = 0; i < b->n_texts; i++) {
+            off[i] = b->h_texts[i].start;
+            off[i + 1] = b->h_texts[i].start + b->h_texts[i].n;
+            fil[i] = b->h_texts[i].filter;
+        }
+        // texts are contiguous ranges of one buffer; rebuild offsets from starts
+        RET_IF(ffbf_batch_set_texts(b, b->texts, off.data(), fil.data(), b->n_texts));
+    }
+    return FFBF_OK;
+}
+
+int ffbf_batch_get_tuning(ffbf_batch *b, ffbf_batch_tuning *t)
+{
+    RET_IF(check_batch(b));
+    if (!t) return fail(FFBF_EINVAL, "null tuning");
+    *t = b->tuning;
+    t->probes = b->probes_eff;
+    return FFBF_OK;
+}
+
+int ffbf_batch_autotune(ffbf_batch *b, ffbf_batch_tuning *best, float *best_ms)
+{
+    RET_IF(check_batch(b));
+    if (!b->have_texts) return fail(FFBF_ESTATE, "autotune needs a batch of texts");
+    const ffbf_batch_tuning start = b->tuning;
+    ffbf_batch_tuning top = start;
+    float top_ms = 1e30f;
+    const uint32_t chunk_opts[] = {16u, 32u, 64u, 128u};
+    const uint32_t block_opts[] = {1024u, 512u};
+    const bool any_lds = b->pre_words_max > 0;
+    for (uint32_t blk : block_opts) {
+        for (uint32_t p = 1; p <= std::min(3u, b->k); p++) {
+            if (!any_lds && p > 1) break;
+            for (uint32_t ci : chunk_opts) {
+                ffbf_batch_tuning t{ci, p, blk, start.use_graph};
+                RET_IF(ffbf_batch_set_tuning(b, &t));
+                uint64_t nm = 0;
+                RET_IF(ffbf_batch_run(b, &nm));  // warm (and capture)
+                RET_IF(ffbf_batch_run(b, &nm));
+                if (b->run_ms < top_ms) {
+                    top_ms = b->run_ms;
+                    top = t;
+                }
+            }
+        }
+    }
+    RET_IF(ffbf_batch_set_tuning(b, &top));
+    if (best) *best = top;
+    if (best_ms) *best_ms = top_ms;
+    return FFBF_OK;
+}
+
+int ffbf_batch_timings(ffbf_batch *b, float *run_ms, float *scan_ms)
+{
+    RET_IF(check_batch(b));
+    if (run_ms) *run_ms = b->run_ms;
+    if (scan_ms) *scan_ms = b->scan_ms;
+    return FFBF_OK;
+}
+
+}  // extern "C"

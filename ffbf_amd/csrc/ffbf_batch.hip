@@ -1,0 +1,188 @@
+// ffbf_batch.hip -- batched scan of many texts against many resident filters
+// in one persistent launch, and the matching verification kernel.
+//
+// The reference scans one text against one filter per run
+// (greptest.cu:505-576).  Here the texts of a batch sit back to back in one
+// device buffer, each routed to one filter; the host cuts every text into
+// chunks of 1 KiB-iterations, ordered by filter, and persistent workgroups
+// pull chunks from a global counter.  A workgroup reloads its LDS prefilter
+// only when the next chunk belongs to another filter, and drains its survivor
+// queues before it does.  Candidates carry their text id in the top bits of
+// the offset.  Per text, the windows evaluated, the hash pairs, the probe
+// cascade and the feed-forward are exactly those of the single-text scan.
+#include "ffbf_scan_common.h"
+
+namespace ffbf {
+
+namespace {
+
+template <int WT, int BLOCK, int P>
+__global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
+{
+    using QG = QueueGeom<BLOCK>;
+    extern __shared__ __align__(16) uint32_t s_pre[];
+    __shared__ uint32_t s_sbox[256];
+    __shared__ uint4 s_q[BLOCK / 64][QG::cap];
+    __shared__ uint32_t s_next;
+
+    for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) s_sbox[i] = b.sbox[i];
+
+    const uint32_t W = WT ? (uint32_t)WT : b.W;
+    const int wv = (int)(threadIdx.x >> 6);
+    const uint32_t n_chunks = (uint32_t)b.ctl[1];
+    uint4 *q = s_q[wv];
+    uint32_t qn = 0;
+    uint32_t loaded = 0xFFFFFFFFu;
+    ScanArgs a{};
+    bool lds = false;
+    constexpr int NCH = WT > 0 ? Chunks<(WT > 0 ? WT : 1)>::NCH : 1;
+
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_next = (uint32_t)atomicAdd(b.ctl, 1ull);
+        __syncthreads();
+        const uint32_t ci = __builtin_amdgcn_readfirstlane(s_next);
+        if (ci >= n_chunks) break;
+        const BatchChunk c = b.chunks[ci];
+        if (c.filter != loaded) {
+            if (qn) {
+                wave_sync();
+                if (lds)
+                    drain<true, P>(q, qn, a, s_pre);
+                else
+                    drain<false, 1>(q, qn, a, s_pre);
+                wave_sync();
+                qn = 0;
+            }
+            a = b.filters[c.filter];
+            lds = a.lds != 0u;
+            __syncthreads();
+            if (lds) {
+                const uint4 *src = (const uint4 *)a.pre;
+                uint4 *dst = (uint4 *)s_pre;
+                for (uint32_t i = threadIdx.x; i < a.pre_words / 4u; i += BLOCK) dst[i] = src[i];
+            }
+            __syncthreads();
+            loaded = c.filter;
+        }
+        const BatchText t = b.tdesc[c.text];
+        const uint8_t *text = b.texts + t.start;
+        const uint64_t lead = (uint64_t)((uintptr_t)text & 15u);
+        const uint8_t *base = text - lead;
+        const uint64_t npos = t.n >= W ? t.n - W + 1 : 0;
+        const uint64_t tag = (uint64_t)c.text << kBatchOffsetBits;
+        const uint64_t it_end = (uint64_t)c.it0 + c.n_it;
+        uint64_t it = (uint64_t)c.it0 + (uint64_t)wv;
+
+        WaveChunks<NCH> cur;
+        if constexpr (WT > 0) {
+            if (it < it_end) cur.load(base, lead, t.n, it << 6);
+        }
+        for (; it < it_end; it += BLOCK / 64) {
+            const uint64_t u = (it << 6) + (uint64_t)lane_id();
+            uint32_t h1[kUnit], h2[kUnit];
+            if constexpr (WT > 0) {
+                WaveChunks<NCH> nxt;
+                if (it + BLOCK / 64 < it_end) nxt.load(base, lead, t.n, (it + BLOCK / 64) << 6);
+                uint4 ch[NCH];
+                cur.spread(ch);
+                hash_chunks<(WT > 0 ? WT : 1)>(ch, s_sbox, h1, h2);
+                cur = nxt;
+            } else {
+                hash_unit_generic(base, lead, t.n, u, W, s_sbox, h1, h2);
+            }
+            const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
+            if (lds)
+                test_and_queue<true, BLOCK, P>(a, s_pre, h1, h2, p0, npos, tag, q, qn);
+            else
+                test_and_queue<false, BLOCK, 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn);
+        }
+    }
+    if (qn) {
+        wave_sync();
+        if (lds)
+            drain<true, P>(q, qn, a, s_pre);
+        else
+            drain<false, 1>(q, qn, a, s_pre);
+    }
+}
+
+// Exact verification of a batch: the candidate's text id selects the text
+// and, through it, the filter whose survivor index and patterns are used.
+__global__ __launch_bounds__(256) void k_verify_batch(BatchVerifyArgs v)
+{
+    const unsigned long long total = v.counters[C_CAND];
+    const uint64_t count = total < v.cand_cap ? total : v.cand_cap;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < count; c += stride) {
+        const Cand e = v.cand[c];
+        const uint32_t tid = (uint32_t)(e.offset >> kBatchOffsetBits);
+        const uint64_t off = e.offset & ((1ull << kBatchOffsetBits) - 1ull);
+        const BatchText t = v.tdesc[tid];
+        const VerifyArgs &f = v.filters[t.filter];
+        verify_candidate(f, v.texts + t.start, t.n, off, e.h1, e.h2, [&](uint32_t id) {
+            const unsigned long long s = atomicAdd(v.counters + C_MATCH, 1ull);
+            if (s < v.out_cap) {
+                BatchMatchRec m;
+                m.text = tid;
+                m.id = id;
+                m.offset = off;
+                v.out[s] = m;
+            }
+        });
+    }
+}
+
+template <int WT, int BLOCK, int P>
+hipError_t launch_batch_t(const BatchArgs &a, uint32_t pre_words_max, int n_cu, hipStream_t s)
+{
+    const size_t dyn = (size_t)pre_words_max * 4u;
+    if (dyn > 0) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan_batch<WT, BLOCK, P>, BLOCK, dyn);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P>), dim3((uint32_t)(n_cu * per_cu)), dim3(BLOCK), dyn, s, a);
+    return hipGetLastError();
+}
+
+template <int WT, int BLOCK>
+hipError_t launch_batch_p(const BatchArgs &a, uint32_t probes, uint32_t pre_words_max, int n_cu, hipStream_t s)
+{
+    if (probes <= 1u) return launch_batch_t<WT, BLOCK, 1>(a, pre_words_max, n_cu, s);
+    if (probes == 2u) return launch_batch_t<WT, BLOCK, 2>(a, pre_words_max, n_cu, s);
+    return launch_batch_t<WT, BLOCK, 3>(a, pre_words_max, n_cu, s);
+}
+
+template <int WT>
+hipError_t launch_batch_w(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
+                          hipStream_t s)
+{
+    if (block == 512u) return launch_batch_p<WT, 512>(a, probes, pre_words_max, n_cu, s);
+    return launch_batch_p<WT, 1024>(a, probes, pre_words_max, n_cu, s);
+}
+
+}  // namespace
+
+hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
+                             hipStream_t s)
+{
+    switch (a.W) {
+    case 19: return launch_batch_w<19>(a, block, probes, pre_words_max, n_cu, s);
+    case 24: return launch_batch_w<24>(a, block, probes, pre_words_max, n_cu, s);
+    case 32: return launch_batch_w<32>(a, block, probes, pre_words_max, n_cu, s);
+    default: return launch_batch_w<0>(a, block, probes, pre_words_max, n_cu, s);
+    }
+}
+
+hipError_t launch_verify_batch(const BatchVerifyArgs &a, int n_cu, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_verify_batch, dim3((uint32_t)(n_cu * 4)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace ffbf

@@ -11,18 +11,19 @@
 #include "ffbf.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
-#include "ffbf_device.h"
+#include "ffbf_ctx.h"
 
 using ffbf::Cand;
 using ffbf::MatchRec;
 
-namespace {
+namespace ffbf {
 
 thread_local std::string g_err;
 
@@ -32,104 +33,12 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess) return fail(FFBF_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define RET_IF(st)                \
-    do {                          \
-        int st_ = (st);           \
-        if (st_ != FFBF_OK) return st_; \
-    } while (0)
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    uint64_t cap = 0;  // elements
-
-    int ensure(uint64_t n, const char *what)
-    {
-        if (n <= cap && p) return FFBF_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        const uint64_t want = std::max<uint64_t>(n, 1);
-        hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            return fail(FFBF_ENOMEM, std::string("hipMalloc ") + what + " (" + std::to_string(want * sizeof(T)) +
-                                         " bytes): " + hipGetErrorString(e));
-        }
-        cap = want;
-        return FFBF_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-enum Ev { EV_B0, EV_B1, EV_S0, EV_S1, EV_S2, EV_P0, EV_P1, EV_V0, EV_V1, EV_N };
-
 uint32_t pow2_at_least(uint64_t n)
 {
     uint32_t l = 0;
     while ((1ull << l) < n) l++;
     return l;
 }
-
-}  // namespace
-
-struct ffbf_ctx {
-    ffbf_params params{};
-    int device = 0;
-    int n_cu = 0;
-    hipStream_t stream = nullptr;
-    uint32_t mask = 0;
-    uint64_t f_words = 0;
-
-    DevBuf<uint32_t> F, R, sbox, fold, gfold;
-    DevBuf<unsigned long long> counters;
-    unsigned long long *h_counters = nullptr;  // pinned
-
-    // scan mode and tuning
-    int mode_req = -1;
-    uint32_t fold_log2_req = 0;
-    uint32_t probes_req = 0, block_req = 0;
-    uint32_t mode = ffbf::MODE_GLOBAL;
-    uint32_t pre_words = 0;  // prefilter size in words (F itself, LDS fold or global fold)
-    uint32_t probes = 1;     // LDS prefilter bits tested before queueing
-    double pre_fill = 0.0;   // fraction of prefilter bits set
-
-    // pattern set
-    DevBuf<uint8_t> bytes;
-    DevBuf<uint64_t> off;
-    DevBuf<uint2> pat_hash;
-    DevBuf<uint32_t> bitmap, surv_ids, table, chain_next;
-    uint64_t n_pat = 0;
-    uint32_t table_size = 0;
-
-    // scan / results
-    const uint8_t *text = nullptr;
-    uint64_t text_n = 0;
-    DevBuf<uint8_t> owned_text;
-    DevBuf<Cand> cand;
-    DevBuf<MatchRec> matches;
-    uint64_t n_cand = 0, n_surv = 0, n_match = 0;
-    std::vector<MatchRec> sorted;
-    bool sorted_valid = false;
-
-    bool patterns = false, built = false, scanned = false, passed = false, verified = false;
-    hipEvent_t ev[EV_N] = {};
-    bool ev_recorded[EV_N] = {};
-};
-
-namespace {
 
 int use_device(ffbf_ctx *c)
 {
@@ -158,18 +67,20 @@ int clear_counters(ffbf_ctx *c, int first, int count)
     return FFBF_OK;
 }
 
-double fill_of(ffbf_ctx *c, const uint32_t *dev, uint64_t words, int *st)
+// Expected fraction of F's bits that are set: exact after a filter load,
+// else k random bits per hashed pattern, 1 - (1 - 1/M)^(k n).
+double f_fill(ffbf_ctx *c)
 {
-    std::vector<uint32_t> h(words);
-    *st = FFBF_OK;
-    if (hipMemcpyAsync(h.data(), dev, words * 4ull, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess) {
-        *st = fail(FFBF_EDEVICE, std::string("prefilter readback: ") + hipGetErrorString(hipGetLastError()));
-        return 0.0;
-    }
-    uint64_t ones = 0;
-    for (uint32_t w : h) ones += (uint64_t)__builtin_popcount(w);
-    return (double)ones / ((double)words * 32.0);
+    const double M = (double)c->f_words * 32.0;
+    if (c->f_set_bits >= 0) return std::min(1.0, (double)c->f_set_bits / M);
+    return 1.0 - std::exp((double)c->params.k * (double)c->n_long * std::log1p(-1.0 / M));
+}
+
+// Expected fill of an OR-fold onto fold_bits bits.
+double fold_fill(ffbf_ctx *c, uint64_t fold_bits)
+{
+    const double ratio = (double)c->f_words * 32.0 / (double)fold_bits;
+    return 1.0 - std::pow(1.0 - f_fill(c), ratio);
 }
 
 // log2 bits of the largest fold that fits one XCD's L2 (4 MiB on MI355X;
@@ -189,7 +100,9 @@ uint32_t l2_fold_log2(ffbf_ctx *c)
 
 // Picks how the scan pre-tests windows, after F changed or a mode was
 // requested.  Automatic choice: the whole filter in LDS when it fits; else
-// the LDS fold when its fill leaves few windows for F; else F itself.
+// the LDS fold when its expected fill leaves few windows for F; else F
+// itself when it fits one XCD's L2, else an L2-sized fold.  The folds are
+// computed on the stream; nothing is read back.
 int choose_mode(ffbf_ctx *c)
 {
     const uint32_t max_words = ffbf::max_fold_words(c->device);
@@ -200,7 +113,7 @@ int choose_mode(ffbf_ctx *c)
         if (c->f_words > max_words) return fail(FFBF_EINVAL, "scan mode 2: filter does not fit in LDS");
         mode = ffbf::MODE_LDS;
         c->pre_words = (uint32_t)c->f_words;
-        c->pre_fill = fill_of(c, c->F.p, c->f_words, &st);
+        c->pre_fill = f_fill(c);
     } else if (c->mode_req == ffbf::MODE_GLOBAL) {
         mode = ffbf::MODE_GLOBAL;
         c->pre_words = (uint32_t)std::min<uint64_t>(c->f_words, 0xFFFFFFFFu);
@@ -224,8 +137,7 @@ int choose_mode(ffbf_ctx *c)
         RET_IF(c->fold.ensure(words, "fold"));
         HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
         c->pre_words = words;
-        c->pre_fill = fill_of(c, c->fold.p, words, &st);
-        RET_IF(st);
+        c->pre_fill = fold_fill(c, (uint64_t)words * 32u);
         // the LDS fold pays while few windows get past its first two bits;
         // otherwise probe F directly if it fits in one XCD's L2, else a fold
         // of F sized to L2
@@ -384,6 +296,23 @@ int do_scan(ffbf_ctx *c)
     return FFBF_OK;
 }
 
+// Sorts the verified matches on the device when they pack into 64-bit keys
+// (ids < 2^24, offsets < 2^40); otherwise the host sorts them on copy-out.
+int sort_on_device(ffbf_ctx *c)
+{
+    c->sorted_on_device = false;
+    if (c->n_match < 2 || !ffbf::sort_packable(c->n_pat, c->text_n)) return FFBF_OK;
+    const int end_bit = ffbf::sort_end_bit(c->n_pat);
+    size_t tb = 0;
+    HIP_TRY(ffbf::sort_temp_bytes(c->n_match, end_bit, &tb));
+    RET_IF(c->sort_keys.ensure(2 * c->n_match, "sort keys"));
+    RET_IF(c->sort_temp.ensure(tb, "sort scratch"));
+    HIP_TRY(ffbf::sort_matches(c->matches.p, c->n_match, c->sort_keys.p, c->sort_keys.p + c->n_match, c->sort_temp.p,
+                               tb, end_bit, c->stream));
+    c->sorted_on_device = true;
+    return FFBF_OK;
+}
+
 int do_verify(ffbf_ctx *c)
 {
     RET_IF(enqueue_verify(c));
@@ -396,7 +325,7 @@ int do_verify(ffbf_ctx *c)
     c->n_match = c->h_counters[ffbf::C_MATCH];
     c->verified = true;
     c->sorted_valid = false;
-    return FFBF_OK;
+    return sort_on_device(c);
 }
 
 int load_sorted_matches(ffbf_ctx *c)
@@ -408,9 +337,10 @@ int load_sorted_matches(ffbf_ctx *c)
                                c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    std::sort(c->sorted.begin(), c->sorted.end(), [](const MatchRec &a, const MatchRec &b) {
-        return a.id != b.id ? a.id < b.id : a.offset < b.offset;
-    });
+    if (!c->sorted_on_device)
+        std::sort(c->sorted.begin(), c->sorted.end(), [](const MatchRec &a, const MatchRec &b) {
+            return a.id != b.id ? a.id < b.id : a.offset < b.offset;
+        });
     c->sorted_valid = true;
     return FFBF_OK;
 }
@@ -426,11 +356,13 @@ float elapsed(ffbf_ctx *c, int a, int b)
     return ms;
 }
 
-}  // namespace
+}  // namespace ffbf
+
+using namespace ffbf;
 
 extern "C" {
 
-const char *ffbf_last_error(void) { return g_err.c_str(); }
+const char *ffbf_last_error(void) { return ffbf::g_err.c_str(); }
 
 const char *ffbf_version(void) { return "ffbf-amd 0.1 (gfx950)"; }
 
@@ -505,6 +437,8 @@ void ffbf_destroy(ffbf_ctx *c)
     c->owned_text.release();
     c->cand.release();
     c->matches.release();
+    c->sort_keys.release();
+    c->sort_temp.release();
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     for (int i = 0; i < EV_N; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -523,6 +457,8 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
         if (offsets[i + 1] < offsets[i]) return fail(FFBF_EINVAL, "offsets must be non-decreasing");
     const uint64_t total = offsets[n];
     if (total && !bytes) return fail(FFBF_EINVAL, "null pattern bytes");
+    uint64_t n_long = 0;
+    for (uint64_t i = 0; i < n; i++) n_long += (offsets[i + 1] - offsets[i] >= c->params.window) ? 1u : 0u;
     RET_IF(c->bytes.ensure(total, "pattern bytes"));
     RET_IF(c->off.ensure(n + 1, "pattern offsets"));
     RET_IF(c->pat_hash.ensure(n, "pattern hashes"));
@@ -540,6 +476,7 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
                                c->pat_hash.p, c->counters.p, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->n_pat = n;
+    c->n_long = n_long;
     c->patterns = true;
     c->scanned = c->passed = c->verified = false;
     return FFBF_OK;
@@ -577,8 +514,8 @@ int ffbf_build(ffbf_ctx *c)
     HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k, c->mask,
                                c->F.p, c->pat_hash.p, c->counters.p, c->stream));
     RET_IF(record(c, EV_B1));
+    c->f_set_bits = -1;
     RET_IF(choose_mode(c));
-    HIP_TRY(hipStreamSynchronize(c->stream));
     c->built = true;
     c->scanned = c->passed = c->verified = false;
     return FFBF_OK;
@@ -590,10 +527,7 @@ int ffbf_set_scan_mode(ffbf_ctx *c, int mode, uint32_t fold_log2_bits)
     if (mode < -1 || mode > 3) return fail(FFBF_EINVAL, "mode must be -1..3");
     c->mode_req = mode;
     c->fold_log2_req = fold_log2_bits;
-    if (c->built) {
-        RET_IF(choose_mode(c));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
+    if (c->built) RET_IF(choose_mode(c));
     return FFBF_OK;
 }
 
@@ -675,6 +609,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     c->n_match = c->h_counters[ffbf::C_MATCH];
     c->verified = true;
     c->sorted_valid = false;
+    RET_IF(sort_on_device(c));
     if (n_matches) *n_matches = c->n_match;
     return FFBF_OK;
 }
@@ -795,6 +730,9 @@ int ffbf_filter_load(ffbf_ctx *c, const uint32_t *deltas, uint64_t n)
         h[prev >> 5] |= 1u << (prev & 31u);
     }
     HIP_TRY(hipMemcpyAsync(c->F.p, h.data(), c->f_words * 4, hipMemcpyHostToDevice, c->stream));
+    uint64_t ones = 0;
+    for (uint32_t w : h) ones += (uint64_t)__builtin_popcount(w);
+    c->f_set_bits = (int64_t)ones;
     RET_IF(choose_mode(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->built = true;

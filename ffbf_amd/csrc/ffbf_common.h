@@ -64,6 +64,18 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// OR of v over the whole wave (uniform result): inclusive OR scan inside each
+// 16-lane row by DPP row shifts, then the four row totals.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v)
+{
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111 /* row_shr:1 */, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112 /* row_shr:2 */, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114 /* row_shr:4 */, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118 /* row_shr:8 */, 0xF, 0xF, true);
+    return __builtin_amdgcn_readlane(v, 15) | __builtin_amdgcn_readlane(v, 31) | __builtin_amdgcn_readlane(v, 47) |
+           __builtin_amdgcn_readlane(v, 63);
+}
+
 // Orders this wave's LDS traffic across its lanes: writes before the call are
 // seen by reads after it, from any lane of the same wave.
 __device__ __forceinline__ void wave_sync()
@@ -131,6 +143,42 @@ __device__ __forceinline__ void append_cand(bool hit, uint64_t off, uint32_t h1,
             c.h2 = h2;
             cand[slot] = c;
         }
+    }
+}
+
+__device__ __forceinline__ bool bytes_equal(const uint8_t *x, const uint8_t *y, uint64_t len)
+{
+    for (uint64_t i = 0; i < len; i++)
+        if (x[i] != y[i]) return false;
+    return true;
+}
+
+// Exact check of one candidate window (offset off of text[0..n), hash pair
+// h1,h2): find the survivors whose prefix-window hash pair is (h1,h2) in the
+// open-addressed index, compare each whole pattern byte for byte, and call
+// emit(id) for every pattern that occurs at off.
+template <class Emit>
+__device__ __forceinline__ void verify_candidate(const VerifyArgs &f, const uint8_t *text, uint64_t n, uint64_t off,
+                                                 uint32_t h1, uint32_t h2, Emit emit)
+{
+    uint32_t slot = key_slot(h1, h2) & f.table_mask;
+    for (;;) {
+        const uint32_t t = f.table[slot];
+        if (t == 0u) return;
+        const uint2 ph = f.pat_hash[t - 1u];
+        if (ph.x == h1 && ph.y == h2) {
+            uint32_t id1 = t;
+            uint32_t next = f.slot_chain[slot];
+            while (id1) {
+                const uint32_t id = id1 - 1u;
+                const uint64_t p0 = f.off[id], len = f.off[id + 1] - p0;
+                if (off + len <= n && bytes_equal(text + off, f.bytes + p0, len)) emit(id);
+                id1 = next;
+                if (next) next = f.chain_next[next - 1u];
+            }
+            return;
+        }
+        slot = (slot + 1u) & f.table_mask;
     }
 }
 

@@ -1,0 +1,119 @@
+// ffbf_ctx.h -- the matcher context behind the C ABI and the helpers its
+// implementation files share (internal).
+#pragma once
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "ffbf.h"
+#include "ffbf_device.h"
+
+namespace ffbf {
+
+// Records msg as the calling thread's last error; returns code.
+int fail(int code, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return ::ffbf::fail(FFBF_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RET_IF(st)                      \
+    do {                                \
+        int st_ = (st);                 \
+        if (st_ != FFBF_OK) return st_; \
+    } while (0)
+
+// A device array that only grows.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    uint64_t cap = 0;  // elements
+
+    int ensure(uint64_t n, const char *what)
+    {
+        if (n <= cap && p) return FFBF_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const uint64_t want = std::max<uint64_t>(n, 1);
+        hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(FFBF_ENOMEM, std::string("hipMalloc ") + what + " (" + std::to_string(want * sizeof(T)) +
+                                         " bytes): " + hipGetErrorString(e));
+        }
+        cap = want;
+        return FFBF_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+enum Ev { EV_B0, EV_B1, EV_S0, EV_S1, EV_S2, EV_P0, EV_P1, EV_V0, EV_V1, EV_N };
+
+uint32_t pow2_at_least(uint64_t n);
+
+}  // namespace ffbf
+
+struct ffbf_ctx {
+    ffbf_params params{};
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    uint32_t mask = 0;
+    uint64_t f_words = 0;
+
+    ffbf::DevBuf<uint32_t> F, R, sbox, fold, gfold;
+    ffbf::DevBuf<unsigned long long> counters;
+    unsigned long long *h_counters = nullptr;  // pinned
+
+    // scan mode and tuning
+    int mode_req = -1;
+    uint32_t fold_log2_req = 0;
+    uint32_t probes_req = 0, block_req = 0;
+    uint32_t mode = ffbf::MODE_GLOBAL;
+    uint32_t pre_words = 0;   // prefilter size in words (F itself, LDS fold or global fold)
+    uint32_t probes = 1;      // LDS prefilter bits tested before queueing
+    double pre_fill = 0.0;    // expected fraction of prefilter bits set
+    uint64_t n_long = 0;      // patterns with length >= W (each sets k bits of F)
+    int64_t f_set_bits = -1;  // exact number of set bits of F when known (filter load)
+
+    // pattern set
+    ffbf::DevBuf<uint8_t> bytes;
+    ffbf::DevBuf<uint64_t> off;
+    ffbf::DevBuf<uint2> pat_hash;
+    ffbf::DevBuf<uint32_t> bitmap, surv_ids, table, chain_next;
+    uint64_t n_pat = 0;
+    uint32_t table_size = 0;
+
+    // scan / results
+    const uint8_t *text = nullptr;
+    uint64_t text_n = 0;
+    ffbf::DevBuf<uint8_t> owned_text;
+    ffbf::DevBuf<ffbf::Cand> cand;
+    ffbf::DevBuf<ffbf::MatchRec> matches;
+    ffbf::DevBuf<uint64_t> sort_keys;  // 2 x capacity
+    ffbf::DevBuf<uint8_t> sort_temp;
+    bool sorted_on_device = false;
+    uint64_t n_cand = 0, n_surv = 0, n_match = 0;
+    std::vector<ffbf::MatchRec> sorted;
+    bool sorted_valid = false;
+
+    bool patterns = false, built = false, scanned = false, passed = false, verified = false;
+    hipEvent_t ev[ffbf::EV_N] = {};
+    bool ev_recorded[ffbf::EV_N] = {};
+};
+
+namespace ffbf {
+// Stage launchers shared with the batch path (ffbf_capi.cpp).
+int enqueue_pass(ffbf_ctx *c);
+int use_device(ffbf_ctx *c);
+}  // namespace ffbf

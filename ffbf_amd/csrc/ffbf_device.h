@@ -50,6 +50,44 @@ struct ScanArgs {
     uint32_t pre_log2w;  // log2(pre_words)
     uint32_t P;          // forward bits tested against the prefilter before queueing (1..k; 1 for global)
     uint32_t pre_exact;  // prefilter is F itself: no further F probes
+    uint32_t lds;        // prefilter is copied to LDS (batched scan: per filter)
+};
+
+// Batched scan: many texts (concatenated in one device buffer), each routed
+// to one of several resident filters, walked by persistent workgroups that
+// pull chunks of 1 KiB-iterations from a global counter.
+struct BatchText {
+    uint64_t start;  // offset of the text in the batch buffer
+    uint64_t n;      // bytes
+    uint32_t filter;
+    uint32_t reserved;
+};
+
+struct BatchChunk {
+    uint32_t text;
+    uint32_t filter;
+    uint32_t it0;   // first 1 KiB iteration of the text's 16-byte units
+    uint32_t n_it;  // iterations in the chunk
+};
+
+struct BatchMatchRec {
+    uint32_t text, id;
+    uint64_t offset;
+};
+static_assert(sizeof(BatchMatchRec) == 16, "BatchMatchRec is 16 bytes");
+
+// Bits of a batched candidate's offset field that hold the offset in its
+// text; the text id sits above them.
+constexpr int kBatchOffsetBits = 40;
+
+struct BatchArgs {
+    const uint8_t *texts;
+    const BatchText *tdesc;
+    const BatchChunk *chunks;
+    const ScanArgs *filters;  // per filter; cand/counters are the batch's
+    const uint32_t *sbox;
+    unsigned long long *ctl;  // [0] next chunk, [1] number of chunks
+    uint32_t W;
 };
 
 struct VerifyArgs {
@@ -67,6 +105,17 @@ struct VerifyArgs {
     MatchRec *out;
     uint64_t out_cap;
     unsigned long long *counters;
+};
+
+struct BatchVerifyArgs {
+    const uint8_t *texts;
+    const BatchText *tdesc;
+    const Cand *cand;
+    uint64_t cand_cap;
+    const VerifyArgs *filters;  // per filter: pattern set and survivor index (text fields unused)
+    BatchMatchRec *out;
+    uint64_t out_cap;
+    unsigned long long *counters;  // C_CAND read, C_MATCH appended
 };
 
 struct PassArgs {
@@ -96,6 +145,19 @@ hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t step
                          hipStream_t s);
 hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
+
+hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
+                             hipStream_t s);
+hipError_t launch_verify_batch(const BatchVerifyArgs &a, int n_cu, hipStream_t s);
+hipError_t sort_batch_matches(BatchMatchRec *m, uint64_t n, uint64_t *keys_a, uint64_t *keys_b, void *temp,
+                              size_t temp_bytes, int id_shift, int text_shift, int end_bit, hipStream_t s);
+
+// Device sort of matches by (id, offset) through packed 64-bit keys.
+bool sort_packable(uint64_t n_patterns, uint64_t text_bytes);
+int sort_end_bit(uint64_t n_patterns);
+hipError_t sort_temp_bytes(uint64_t n, int end_bit, size_t *bytes);
+hipError_t sort_matches(MatchRec *m, uint64_t n, uint64_t *keys_a, uint64_t *keys_b, void *temp, size_t temp_bytes,
+                        int end_bit, hipStream_t s);
 
 // Largest fold (in 32-bit words) the scan kernel can hold in LDS on this device.
 uint32_t max_fold_words(int device);

@@ -134,13 +134,6 @@ __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
     }
 }
 
-__device__ __forceinline__ bool bytes_equal(const uint8_t *x, const uint8_t *y, uint64_t len)
-{
-    for (uint64_t i = 0; i < len; i++)
-        if (x[i] != y[i]) return false;
-    return true;
-}
-
 // Exact verification: each candidate window looks its hash pair up among the
 // survivors' prefix pairs, then compares the whole pattern byte for byte.
 __global__ __launch_bounds__(256) void k_verify(VerifyArgs a)
@@ -150,34 +143,16 @@ __global__ __launch_bounds__(256) void k_verify(VerifyArgs a)
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < count; c += stride) {
         const Cand e = a.cand[c];
-        uint32_t slot = key_slot(e.h1, e.h2) & a.table_mask;
-        for (;;) {
-            const uint32_t t = a.table[slot];
-            if (t == 0u) break;
-            const uint2 ph = a.pat_hash[t - 1u];
-            if (ph.x == e.h1 && ph.y == e.h2) {
-                uint32_t id1 = t;
-                uint32_t next = a.slot_chain[slot];
-                while (id1) {
-                    const uint32_t id = id1 - 1u;
-                    const uint64_t p0 = a.off[id], len = a.off[id + 1] - p0;
-                    if (e.offset + len <= a.n && bytes_equal(a.text + e.offset, a.bytes + p0, len)) {
-                        const unsigned long long s = atomicAdd(a.counters + C_MATCH, 1ull);
-                        if (s < a.out_cap) {
-                            MatchRec m;
-                            m.id = id;
-                            m.reserved = 0u;
-                            m.offset = e.offset;
-                            a.out[s] = m;
-                        }
-                    }
-                    id1 = next;
-                    if (next) next = a.chain_next[next - 1u];
-                }
-                break;
+        verify_candidate(a, a.text, a.n, e.offset, e.h1, e.h2, [&](uint32_t id) {
+            const unsigned long long s = atomicAdd(a.counters + C_MATCH, 1ull);
+            if (s < a.out_cap) {
+                MatchRec m;
+                m.id = id;
+                m.reserved = 0u;
+                m.offset = e.offset;
+                a.out[s] = m;
             }
-            slot = (slot + 1u) & a.table_mask;
-        }
+        });
     }
 }
 

@@ -243,13 +243,12 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         for (int b = 0; b < kUnit; b++) pm |= ((fw[b] >> (h1[b] & 31u)) & 1u) << b;
     }
     pm &= vm;
-    // one drain site: b is wave-uniform, so h1[b]/h2[b] index registers
-    // directly (no scratch) and the loop stays rolled
-#pragma unroll 1
-    for (uint32_t b = 0; b < (uint32_t)kUnit; b++) {
+    // visit only the slots where some lane passed; one drain site.  b is
+    // wave-uniform, so h1[b]/h2[b] index registers directly (no scratch)
+    for (uint32_t any = wave_or(pm); any; any &= any - 1u) {
+        const uint32_t b = (uint32_t)__builtin_ctz(any);
         const bool pass = (pm >> b) & 1u;
         const uint64_t bal = __ballot(pass);
-        if (!bal) continue;
         if (pass) {
             const uint64_t p = (uint64_t)(p0 + (int64_t)b) | tag;
             q[qn + lanes_below(bal)] = make_uint4((uint32_t)p, (uint32_t)(p >> 32), h1[b], h2[b]);

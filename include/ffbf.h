@@ -171,6 +171,52 @@ int ffbf_synchronize(ffbf_ctx *ctx);
  * device work against the matcher's. */
 void *ffbf_get_stream(ffbf_ctx *ctx);
 
+/* ---- Batched scans: many texts, many resident filters, one launch ----
+ * The reference scans one text against one filter per run
+ * (greptest.cu:505-576, ffbf.cpp:310-392); a batch binds several built
+ * contexts (same device, window, k and sbox) and scans a set of device texts,
+ * each routed to one of them, in one persistent launch replayed from a
+ * captured hipGraph.  Each text's match list equals what ffbf_run_device
+ * gives for it alone.  A filter's R after a batch run is the union over the
+ * texts routed to it.  While a batch is in use its contexts must not be used
+ * directly. */
+typedef struct ffbf_batch ffbf_batch;
+
+typedef struct ffbf_batch_match {
+    uint32_t text_id;
+    uint32_t pattern_id;
+    uint64_t offset; /* within the text */
+} ffbf_batch_match;
+
+typedef struct ffbf_batch_tuning {
+    uint32_t chunk_iters; /* 1 KiB iterations per work chunk (0 = 32) */
+    uint32_t probes;      /* LDS prefilter bits tested before queueing, 1..min(k,3) (0 = auto) */
+    uint32_t block;       /* workgroup size, 512 or 1024 (0 = 1024) */
+    uint32_t use_graph;   /* 1: replay the captured hipGraph; 0: launch stage by stage */
+} ffbf_batch_tuning;
+
+int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch **out);
+void ffbf_batch_destroy(ffbf_batch *batch);
+/* Texts are dev_texts[offsets[i] .. offsets[i+1]) (offsets has n_texts+1
+ * entries), text i routed to filter filter_of_text[i] (index into the
+ * create-time array).  The device buffer is borrowed until the next call. */
+int ffbf_batch_set_texts(ffbf_batch *batch, const uint8_t *dev_texts, const uint64_t *offsets,
+                         const uint32_t *filter_of_text, uint32_t n_texts);
+int ffbf_batch_run(ffbf_batch *batch, uint64_t *n_matches);
+/* The same texts through the single-text path, one ffbf_run_device each
+ * (the per-request launch baseline); results land in the same match list. */
+int ffbf_batch_run_per_text(ffbf_batch *batch, uint64_t *n_matches);
+/* Matches sorted by (text id, pattern id, offset). */
+int ffbf_batch_get_matches(ffbf_batch *batch, ffbf_batch_match *out, uint64_t cap, uint64_t *n);
+int ffbf_batch_set_tuning(ffbf_batch *batch, const ffbf_batch_tuning *tuning);
+int ffbf_batch_get_tuning(ffbf_batch *batch, ffbf_batch_tuning *tuning);
+/* Times every (block, probes, chunk size) candidate on the current texts and
+ * keeps the fastest; results never depend on the tuning. */
+int ffbf_batch_autotune(ffbf_batch *batch, ffbf_batch_tuning *best, float *best_ms);
+/* Device time of the last run (whole graph) and of its scan kernel (only
+ * when the run was not a graph replay; else -1). */
+int ffbf_batch_timings(ffbf_batch *batch, float *run_ms, float *scan_ms);
+
 /* Plain device-memory helpers for callers without their own HIP code. */
 int ffbf_device_alloc(int device, uint64_t bytes, void **out);
 int ffbf_device_free(void *ptr);

@@ -1,0 +1,176 @@
+"""Batched scan (many texts x many resident filters, one persistent launch,
+hipGraph replay) against the oracle text by text, against the per-text
+launch path, and across every tuning.  Needs an MI355X (-m gpu)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import bits_of
+
+import ffbf_amd as fa
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+W, K = 24, 3
+
+
+def make_filters(sbox, specs, seed):
+    """specs: list of (n_patterns, log2_bits, alphabet); returns matchers and pattern sets."""
+    out = []
+    for i, (n, log2, alpha) in enumerate(specs):
+        b, off = fa.gen_patterns(alpha, seed + i, n, W, 60)
+        m = fa.Matcher(W, K, log2, sbox)
+        m.set_patterns(b, off)
+        m.build()
+        out.append((m, b, off, log2))
+    return out
+
+
+def make_texts(filters, n_texts, seed, max_len=200_000):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, max_len, size=n_texts)
+    lens[:4] = [0, 5, W - 1, W]  # empty and too-short texts
+    route = rng.integers(0, len(filters), size=n_texts).astype(np.uint32)
+    texts = []
+    for i, (n, f) in enumerate(zip(lens.tolist(), route.tolist())):
+        alpha = fa.ALPHA_DNA_GC40 if f % 2 == 0 else fa.ALPHA_ASCII
+        t = fa.gen_bytes(alpha, seed * 1000 + i, n)
+        _, b, off, _ = filters[f]
+        if n > 100:
+            fa.plant(seed * 7 + i, 1 + n // 20000, t, b, off)
+        texts.append(t)
+    off = np.zeros(n_texts + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(lens)
+    return texts, off, route
+
+
+def oracle_batch(sbox, filters, texts, route):
+    """Sorted (text, pattern, offset) rows and, per filter, the OR of the
+    reverse filters of the texts routed to it."""
+    rows, R = [], {}
+    for i, t in enumerate(texts):
+        f = int(route[i])
+        _, b, off, log2 = filters[f]
+        r = Oracle(W, K, log2, sbox).run(t, b, off)
+        rows += [(i, int(p), int(q)) for p, q in zip(r["ids"], r["offs"])]
+        R[f] = r["R"] if f not in R else (R[f] | r["R"])
+    return sorted(rows), R
+
+
+def as_rows(m):
+    return list(zip(m["text_id"].tolist(), m["pattern_id"].tolist(), m["offset"].tolist()))
+
+
+SPECS = [
+    (500, 14, fa.ALPHA_DNA_GC40),      # whole filter in LDS
+    (3000, 22, fa.ALPHA_ASCII),        # LDS fold
+    (20000, 18, fa.ALPHA_DNA_GC40),    # saturated-ish: F probed in L2
+    (60000, 26, fa.ALPHA_ASCII),       # L2-sized global fold
+]
+
+
+def test_batch_equals_oracle_and_per_text():
+    say("batch vs oracle")
+    sbox = fa.sbox_from_seed(77)
+    filters = make_filters(sbox, SPECS, 300)
+    modes = [f[0].timings()["scan_mode"] for f in filters]
+    say(f"  filter modes {modes}")
+    texts, off, route = make_texts(filters, 60, 5)
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts) if texts else np.zeros(1, np.uint8))
+    expect, R = oracle_batch(sbox, filters, texts, route)
+    assert len(expect) > 20
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        n = bt.run()
+        got = as_rows(bt.matches())
+        assert n == len(expect)
+        assert got == expect
+        # a filter's R after the batch is the union over its texts
+        for f, (m, _, _, _) in enumerate(filters):
+            if f in R:
+                assert np.array_equal(m.filter(fa.FILTER_REVERSE), R[f]), f"R of filter {f}"
+        n2 = bt.run_per_text()
+        assert as_rows(bt.matches()) == expect and n2 == n
+        # a second replay of the same graph gives the same answer
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+    dev.free()
+    for f in filters:
+        f[0].close()
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(chunk_iters=1, probes=1, block=1024, use_graph=True),
+    dict(chunk_iters=7, probes=2, block=512, use_graph=False),
+    dict(chunk_iters=64, probes=3, block=1024, use_graph=True),
+    dict(chunk_iters=1000, probes=0, block=0, use_graph=False),
+])
+def test_batch_tunings_agree(tuning):
+    say(f"batch tuning {tuning}")
+    sbox = fa.sbox_from_seed(78)
+    filters = make_filters(sbox, SPECS[:3], 400)
+    texts, off, route = make_texts(filters, 40, 6, max_len=120_000)
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        bt.run()
+        base = as_rows(bt.matches())
+        bt.set_tuning(**tuning)
+        bt.run()
+        assert as_rows(bt.matches()) == base
+        t = bt.tuning()
+        assert t["block"] in (512, 1024)
+    dev.free()
+    for f in filters:
+        f[0].close()
+
+
+def test_batch_autotune_and_new_texts():
+    say("batch autotune")
+    sbox = fa.sbox_from_seed(79)
+    filters = make_filters(sbox, SPECS[:2], 500)
+    texts, off, route = make_texts(filters, 30, 7)
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    expect, _ = oracle_batch(sbox, filters, texts, route)
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        best, ms = bt.autotune()
+        assert ms > 0 and best["chunk_iters"] in (16, 32, 64, 128)
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+        # a different batch on the same object (graph reused)
+        texts2, off2, route2 = make_texts(filters, 25, 8)
+        dev2 = fa.DeviceBuffer.from_host(np.concatenate(texts2))
+        expect2, _ = oracle_batch(sbox, filters, texts2, route2)
+        bt.set_texts(dev2.ptr, off2, route2)
+        bt.run()
+        assert as_rows(bt.matches()) == expect2
+        dev2.free()
+    dev.free()
+    for f in filters:
+        f[0].close()
+
+
+def test_batch_candidate_overflow():
+    say("batch overflow")
+    sbox = fa.sbox_from_seed(80)
+    # a saturated 1 Kibit filter: almost every window is a candidate
+    filters = make_filters(sbox, [(2000, 10, fa.ALPHA_DNA_GC40), (100, 14, fa.ALPHA_ASCII)], 600)
+    texts, off, route = make_texts(filters, 12, 9, max_len=150_000)
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    expect, _ = oracle_batch(sbox, filters, texts, route)
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+    dev.free()
+    for f in filters:
+        f[0].close()
