@@ -67,6 +67,138 @@ def make_inputs(fa, wl):
     return sbox, pb, po, text
 
 
+def next_pow2(x: int) -> int:
+    p = 1
+    while p < x:
+        p <<= 1
+    return p
+
+
+def batch_inputs(fa, name, scale=1.0):
+    """Synthetic batch of texts routed to resident pattern sets.
+
+    batch3: 4,096 texts, log-normal sizes (median 256 KiB, sigma 0.9, clipped
+    to 4 KiB-64 MiB, ~1.5 GiB), printable ASCII, one 100,000-pattern set
+    (lengths 20-64), W=19, k=4, 2^26-bit filter; one planted occurrence per
+    started MiB of text.
+    batch4: 64 pattern sets with n log-uniform in [1e3, 1e6] (lengths 24-80,
+    DNA), 2,048 DNA texts (log-normal, median 1 MiB, sigma 1.0, clipped to
+    4 KiB-64 MiB) with one tandem-repeat run (period 1-6, 205-614 bases, ~10%
+    of bases) per 4 KiB, each text routed to a seeded set; W=24, k=3,
+    filter bits = 16 n rounded up to a power of two; one plant per started MiB.
+    """
+    import numpy as np
+
+    if name == "batch3":
+        W, k, seed = 19, 4, 31
+        n_texts = max(8, int(4096 * scale))
+        sizes = fa.lognormal_sizes(seed, n_texts, 256 * 1024.0, 0.9, 4096, 64 << 20)
+        sets = [(100_000, 20, 64, 26, fa.ALPHA_ASCII)]
+        route = np.zeros(n_texts, dtype=np.uint32)
+        alpha_of = lambda f: fa.ALPHA_ASCII
+        repeats = False
+    elif name == "batch4":
+        W, k, seed = 24, 3, 41
+        n_texts = max(8, int(2048 * scale))
+        sizes = fa.lognormal_sizes(seed, n_texts, 1024 * 1024.0, 1.0, 4096, 64 << 20)
+        sets = []
+        for f in range(64):
+            u = (fa.gen_u64(seed, 21, f) >> 11) / float(1 << 53)
+            n = int(round(1000.0 * 1000.0 ** u))
+            sets.append((n, 24, 80, max(10, next_pow2(16 * n).bit_length() - 1), fa.ALPHA_DNA_GC40))
+        route = np.array([fa.gen_u64(seed, 22, i) % 64 for i in range(n_texts)], dtype=np.uint32)
+        alpha_of = lambda f: fa.ALPHA_DNA_GC40
+        repeats = True
+    else:
+        raise ValueError(name)
+    sbox = fa.sbox_from_seed(seed)
+    psets = []
+    for f, (n, lmin, lmax, log2, alpha) in enumerate(sets):
+        pb, po = fa.gen_patterns(alpha, seed * 100 + f, n, lmin, lmax)
+        psets.append((pb, po, log2))
+    off = np.zeros(n_texts + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(sizes)
+    buf = np.empty(int(off[-1]), dtype=np.uint8)
+    for i in range(n_texts):
+        a, b = int(off[i]), int(off[i + 1])
+        t = buf[a:b]
+        t[:] = fa.gen_bytes(alpha_of(route[i]), seed * 1000 + 1, b - a, start=a)
+        if repeats:
+            fa.tandem_repeats(seed * 1000 + 2 + i, t, block=4096)
+        pb, po, _ = psets[route[i]]
+        fa.plant(seed * 1000 + 3 + i, (b - a + (1 << 20) - 1) >> 20, t, pb, po)
+    return {"W": W, "k": k, "sbox": sbox, "sets": psets, "texts": buf, "offsets": off, "route": route,
+            "n_texts": n_texts, "bytes": int(off[-1])}
+
+
+def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0):
+    """Batched (one persistent launch per batch, hipGraph replay, autotuned)
+    vs the per-text launch path on the same texts; GB/s of text for each."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    d = batch_inputs(fa, name, scale)
+    log(f"{name}: {d['n_texts']} texts, {d['bytes'] / 2**30:.2f} GiB, {len(d['sets'])} pattern sets, "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    dev = fa.DeviceBuffer.from_host(d["texts"])
+    ms = []
+    for pb, po, log2 in d["sets"]:
+        m = fa.Matcher(d["W"], d["k"], log2, d["sbox"])
+        m.set_patterns(pb, po)
+        m.build()
+        ms.append(m)
+    bt = fa.Batch(ms)
+    bt.set_texts(dev.ptr, d["offsets"], d["route"])
+    t0 = time.perf_counter()
+    best, best_ms = bt.autotune()
+    tune_s = time.perf_counter() - t0
+    bt.run()
+    ref = bt.matches()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bt.run()
+        got = bt.matches()
+    batched_s = (time.perf_counter() - t0) / steps
+    run_ms, _ = bt.timings()
+    # scan kernel alone: one stage-by-stage run
+    bt.set_tuning(best["chunk_iters"], best["probes"], best["block"], use_graph=False)
+    bt.run()
+    _, scan_ms = bt.timings()
+    bt.set_tuning(best["chunk_iters"], best["probes"], best["block"], use_graph=True)
+    bt.run_per_text()
+    t0 = time.perf_counter()
+    for _ in range(per_text_reps):
+        bt.run_per_text()
+        per = bt.matches()
+    per_s = (time.perf_counter() - t0) / per_text_reps
+    same = bool(np.array_equal(per, ref) and np.array_equal(got, ref))
+    bt.close()
+    for m in ms:
+        m.close()
+    dev.free()
+    gbs = d["bytes"] / batched_s / 1e9
+    per_gbs = d["bytes"] / per_s / 1e9
+    return {
+        "workload": name,
+        "texts": d["n_texts"],
+        "text_bytes": d["bytes"],
+        "pattern_sets": len(d["sets"]),
+        "value": round(gbs, 3),
+        "unit": "GB/s",
+        "ms_per_batch": round(batched_s * 1e3, 3),
+        "graph_run_ms": round(run_ms, 3),
+        "scan_kernel_ms": round(scan_ms, 3),
+        "scan_kernel_gbs": round(d["bytes"] / (scan_ms * 1e-3) / 1e9, 3) if scan_ms > 0 else None,
+        "per_text_gbs": round(per_gbs, 3),
+        "per_text_ms": round(per_s * 1e3, 3),
+        "speedup_vs_per_text": round(gbs / per_gbs, 2),
+        "matches": int(ref.size),
+        "same_matches_as_per_text": same,
+        "autotuned": best,
+        "autotune_s": round(tune_s, 2),
+    }
+
+
 def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
     """Oracle (C restatement of the reference path) on a bounded text sample."""
     import numpy as np
@@ -139,7 +271,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS) + ["batch3", "batch4"])
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--mode", type=int, default=-1, help="scan mode override (-1 auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -163,6 +296,17 @@ def main() -> int:
         for _ in range(3):
             dst.copy_from(src, n)
         print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return 0
+
+    if args.workload in ("batch3", "batch4"):
+        r = run_batched(fa, args.workload, steps=args.steps)
+        print(json.dumps({"metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": r["ms_per_batch"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+                          "data": "synthetic (seeded counter-based generator)",
+                          "config": {"workload": r["workload"], "texts": r["texts"], "text_bytes": r["text_bytes"],
+                                     "pattern_sets": r["pattern_sets"]},
+                          "batched": r}), flush=True)
         return 0
 
     wl = WORKLOADS[args.workload]
@@ -228,13 +372,17 @@ def main() -> int:
         },
         "stages_ms": {"build": round(st[0], 4), "scan": round(st[1], 4), "pattern_pass": round(st[2], 4),
                       "verify": round(st[3], 4)},
-        "scan_mode": {0: "global first probe", 1: "LDS fold prefilter", 2: "filter in LDS"}.get(mode["scan_mode"]),
+        "scan_mode": {0: "F probed in L2/HBM", 1: "LDS fold prefilter", 2: "filter in LDS",
+                      3: "L2-resident global fold"}.get(mode["scan_mode"]),
         "fold_log2_bits": mode["fold_log2_bits"],
         "matches": int(res.size),
         "candidates": int(m.n_candidates()),
     }
     m.close()
     dev.free()
+    if args.workload == "plain" and not args.no_batched:
+        log("batched leg (batch3) ...")
+        out["batched"] = run_batched(fa, "batch3", steps=10)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, sbox, pb, po, text)

@@ -152,6 +152,7 @@ ABI = {
     "ffbf_gen_tandem_repeats": (_i32, [_u64, _P, _u64, _u32, _u32, _u32, _u32, _u32, _i32]),
     "ffbf_gen_sbox": (_i32, [_u64, _P]),
     "ffbf_gen_u64": (_u64, [_u64, _u64, _u64]),
+    "ffbf_gen_lognormal_sizes": (_i32, [_u64, _u64, _c.c_double, _c.c_double, _u64, _u64, _P]),
 }
 
 for _name, (_res, _args) in ABI.items():
@@ -227,6 +228,12 @@ def tandem_repeats(seed: int, text: np.ndarray, block: int = 4096, min_len: int 
                    min_period: int = 1, max_period: int = 6, threads: int = 0) -> None:
     _check(_lib.ffbf_gen_tandem_repeats(seed, _ptr(text), text.size, block, min_len, max_len, min_period, max_period,
                                         threads))
+
+
+def lognormal_sizes(seed: int, n: int, median: float, sigma: float, lo: int, hi: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint64)
+    _check(_lib.ffbf_gen_lognormal_sizes(seed, n, median, sigma, lo, hi, _ptr(out)))
+    return out
 
 
 def gen_u64(seed: int, tag: int, i: int) -> int:

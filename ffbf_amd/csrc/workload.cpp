@@ -8,6 +8,7 @@
 #include "ffbf_workload.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -94,9 +95,48 @@ void parallel_for(uint64_t n, int nthreads, F fn)
     for (auto &t : ts) t.join();
 }
 
+// exp(x) from IEEE-exact pieces only (floor, ldexp, +, *, /): range reduction
+// by ln 2, then the Taylor series of exp(r) for |r| <= ln2/2 to degree 16.
+// Deterministic on every machine, unlike a libm or SIMD exp.
+double det_exp(double x)
+{
+    const double ln2 = 0.69314718055994530942;
+    const double kf = std::floor(x / ln2 + 0.5);
+    const double r = x - kf * ln2;
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i <= 16; i++) {
+        term = term * r / (double)i;
+        sum += term;
+    }
+    return std::ldexp(sum, (int)kf);
+}
+
+// Standard normal approximated by the Irwin-Hall sum of 12 uniforms minus 6
+// (uniforms from 53-bit draws; exact arithmetic).
+double approx_normal(uint64_t key, uint64_t i)
+{
+    double s = 0.0;
+    for (uint64_t j = 0; j < 12; j++) s += (double)(draw(key, 12 * i + j) >> 11) * (1.0 / 9007199254740992.0);
+    return s - 6.0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ffbf_gen_lognormal_sizes(uint64_t seed, uint64_t n, double median, double sigma, uint64_t lo, uint64_t hi,
+                             uint64_t *out)
+{
+    if (!out || lo > hi || median <= 0.0 || sigma < 0.0) return -1;
+    const uint64_t key = stream_key(seed, TAG_SIZES);
+    for (uint64_t i = 0; i < n; i++) {
+        const double v = median * det_exp(sigma * approx_normal(key, i));
+        uint64_t x = v >= 1.8e19 ? UINT64_MAX : (uint64_t)(v + 0.5);
+        out[i] = std::min(hi, std::max(lo, x));
+    }
+    return 0;
+}
+
 
 int ffbf_gen_bytes(uint32_t alphabet, uint64_t seed, uint64_t start, uint64_t n, uint8_t *out, int nthreads)
 {

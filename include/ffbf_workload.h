@@ -50,6 +50,12 @@ int ffbf_gen_tandem_repeats(uint64_t seed, uint8_t *text, uint64_t n, uint32_t b
 /* 256 seeded random uint32 substitution values. */
 int ffbf_gen_sbox(uint64_t seed, uint32_t *sbox);
 
+/* n log-normal sizes: median * exp(sigma * Z), Z the Irwin-Hall
+ * approximation of a standard normal, rounded and clipped to [lo, hi].
+ * Uses no libm, so every machine produces the same sizes. */
+int ffbf_gen_lognormal_sizes(uint64_t seed, uint64_t n, double median, double sigma, uint64_t lo, uint64_t hi,
+                             uint64_t *out);
+
 /* Raw draw i of the stream (seed, tag): used for sizes and routing. */
 uint64_t ffbf_gen_u64(uint64_t seed, uint64_t tag, uint64_t i);
 
