@@ -144,6 +144,12 @@ ABI = {
     "ffbf_batch_get_tuning": (_i32, [_P, _c.POINTER(BatchTuning)]),
     "ffbf_batch_autotune": (_i32, [_P, _c.POINTER(BatchTuning), _c.POINTER(_c.c_float)]),
     "ffbf_batch_timings": (_i32, [_P, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float)]),
+    "ffbf_server_create": (_i32, [_P, _u32, _P, _c.POINTER(_P)]),
+    "ffbf_server_destroy": (None, [_P]),
+    "ffbf_server_submit": (_i32, [_P, _u32, _P, _u64, _pu64]),
+    "ffbf_server_poll": (_i32, [_P, _u64, _c.POINTER(_c.c_int)]),
+    "ffbf_server_wait": (_i32, [_P, _u64, _c.POINTER(_P), _pu64, _c.POINTER(_c.c_double)]),
+    "ffbf_server_get_stats": (_i32, [_P, _P]),
     # workload generators (host only)
     "ffbf_gen_bytes": (_i32, [_u32, _u64, _u64, _u64, _P, _i32]),
     "ffbf_gen_pattern_offsets": (_i32, [_u64, _u64, _u32, _u32, _P]),
@@ -536,3 +542,86 @@ class Batch:
         scan = _c.c_float(0)
         _check(_lib.ffbf_batch_timings(self._h, _c.byref(run), _c.byref(scan)))
         return run.value, scan.value
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("max_batch_bytes", ctypes.c_uint64),
+        ("slots", ctypes.c_uint32),
+        ("max_batch_requests", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("submitted", "completed", "rejected", "batches", "bytes", "pending", "in_flight_bytes")]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Server:
+    """Continuous-batching request loop over resident filters (copies of the
+    given Matchers).  submit() returns a request id; wait() blocks for its
+    matches and the submit-to-completion latency."""
+
+    def __init__(self, matchers: Sequence[Matcher], hbm_budget_bytes: int = 0, slots: int = 0,
+                 max_batch_requests: int = 0, max_batch_bytes: int = 0):
+        arr = (_c.c_void_p * len(matchers))(*[m._h.value for m in matchers])
+        cfg = ServerConfig(hbm_budget_bytes, max_batch_bytes, slots, max_batch_requests)
+        h = _c.c_void_p(0)
+        _check(_lib.ffbf_server_create(arr, len(matchers), _c.byref(cfg), _c.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, filter_id: int, text) -> int:
+        t = np.frombuffer(text, dtype=np.uint8) if isinstance(text, (bytes, bytearray)) else np.ascontiguousarray(text)
+        rid = _c.c_uint64(0)
+        _check(_lib.ffbf_server_submit(self._h, filter_id, _ptr(t), t.size, _c.byref(rid)))
+        self._keep[rid.value] = t  # the text must outlive the request
+        return rid.value
+
+    def poll(self, rid: int) -> bool:
+        d = _c.c_int(0)
+        _check(_lib.ffbf_server_poll(self._h, rid, _c.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid: int):
+        out = _c.c_void_p(0)
+        n = _c.c_uint64(0)
+        lat = _c.c_double(0)
+        st = _lib.ffbf_server_wait(self._h, rid, _c.byref(out), _c.byref(n), _c.byref(lat))
+        self._keep.pop(rid, None)
+        _check(st)
+        try:
+            if n.value:
+                raw = (_c.c_uint8 * (n.value * MATCH_DTYPE.itemsize)).from_address(out.value)
+                arr = np.frombuffer(raw, dtype=MATCH_DTYPE).copy()
+            else:
+                arr = np.zeros(0, dtype=MATCH_DTYPE)
+        finally:
+            _lib.ffbf_free_matches(out)
+        return arr, lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib.ffbf_server_get_stats(self._h, _c.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.ffbf_server_destroy(self._h)
+            self._h = _c.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -217,6 +217,44 @@ int ffbf_batch_autotune(ffbf_batch *batch, ffbf_batch_tuning *best, float *best_
  * when the run was not a graph replay; else -1). */
 int ffbf_batch_timings(ffbf_batch *batch, float *run_ms, float *scan_ms);
 
+/* ---- Serving: continuous batching over resident filters ----
+ * The reference has no request loop.  A server keeps `slots` copies of the
+ * given built contexts resident (each copy with its own R and survivor
+ * index), and one worker per slot: a free slot admits, in arrival order,
+ * every pending request that fits its text buffer and the HBM budget, stages
+ * the texts, scans them as one batch on its own stream and completes them;
+ * slots overlap on the device.  A request's matches equal what
+ * ffbf_match_text gives for its text alone.  The contexts passed in are only
+ * read at creation. */
+typedef struct ffbf_server ffbf_server;
+
+typedef struct ffbf_server_config {
+    uint64_t hbm_budget_bytes;   /* device bytes of request data in flight (0 = 4 GiB) */
+    uint64_t max_batch_bytes;    /* text bytes per batch = per slot buffer (0 = budget / slots) */
+    uint32_t slots;              /* batches on the device at once (0 = 2) */
+    uint32_t max_batch_requests; /* requests per batch (0 = 512) */
+} ffbf_server_config;
+
+typedef struct ffbf_server_stats {
+    uint64_t submitted, completed, rejected, batches, bytes, pending, in_flight_bytes;
+} ffbf_server_stats;
+
+int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_server_config *config,
+                       ffbf_server **out);
+/* Finishes every request already submitted, then stops.  Call after the
+ * last ffbf_server_wait. */
+void ffbf_server_destroy(ffbf_server *server);
+/* Queues a scan of text[0..n) (host memory, borrowed until the request
+ * completes) against filter `filter`.  FFBF_ECAPACITY if the request can
+ * never be admitted (larger than a slot or the budget). */
+int ffbf_server_submit(ffbf_server *server, uint32_t filter, const uint8_t *text, uint64_t n, uint64_t *request_id);
+int ffbf_server_poll(ffbf_server *server, uint64_t request_id, int *done);
+/* Blocks until the request completes; *matches (sorted by pattern id, then
+ * offset) is allocated by the library (ffbf_free_matches); latency_ms is
+ * submit-to-completion.  The request id is released. */
+int ffbf_server_wait(ffbf_server *server, uint64_t request_id, ffbf_match **matches, uint64_t *n, double *latency_ms);
+int ffbf_server_get_stats(ffbf_server *server, ffbf_server_stats *out);
+
 /* Plain device-memory helpers for callers without their own HIP code. */
 int ffbf_device_alloc(int device, uint64_t bytes, void **out);
 int ffbf_device_free(void *ptr);

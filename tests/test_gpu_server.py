@@ -1,0 +1,117 @@
+"""Serving loop: every request's matches in the mix (continuous batching,
+two slots overlapping on the device) equal its matches scanned alone, and
+admission by HBM footprint rejects what can never fit.  Needs an MI355X."""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import ffbf_amd as fa
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+W, K = 24, 3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_filters(sbox):
+    specs = [(800, 14, fa.ALPHA_DNA_GC40), (5000, 17, fa.ALPHA_ASCII), (40000, 20, fa.ALPHA_DNA_GC40)]
+    out = []
+    for i, (n, log2, alpha) in enumerate(specs):
+        b, off = fa.gen_patterns(alpha, 900 + i, n, W, 70)
+        m = fa.Matcher(W, K, log2, sbox)
+        m.set_patterns(b, off)
+        m.build()
+        out.append((m, b, off, log2, alpha))
+    return out
+
+
+def make_requests(filters, n, seed, max_len):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        f = int(rng.integers(0, len(filters)))
+        size = int(rng.integers(0, max_len)) if i % 7 else int(rng.integers(0, 64))
+        _, b, off, _, alpha = filters[f]
+        t = fa.gen_bytes(alpha, seed * 100 + i, size)
+        if size > 200:
+            fa.plant(seed * 31 + i, 1 + size // 30000, t, b, off)
+        reqs.append((f, t))
+    return reqs
+
+
+def alone(filters, f, t):
+    return filters[f][0].match(t).matches
+
+
+def test_server_results_equal_alone():
+    say("server mix vs alone")
+    sbox = fa.sbox_from_seed(91)
+    filters = make_filters(sbox)
+    reqs = make_requests(filters, 80, 3, 300_000)
+    expect = [alone(filters, f, t) for f, t in reqs]
+    # spot-check the reference answer itself against the oracle
+    for i in (1, 2, 3):
+        f, t = reqs[i]
+        _, b, off, log2, _ = filters[f]
+        r = Oracle(W, K, log2, sbox).run(t, b, off)
+        assert np.array_equal(expect[i]["pattern_id"], r["ids"]) and np.array_equal(expect[i]["offset"], r["offs"])
+    assert sum(e.size for e in expect) > 20
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=64 << 20, slots=2) as srv:
+        # all at once
+        ids = [srv.submit(f, t) for f, t in reqs]
+        for i, rid in enumerate(ids):
+            got, lat = srv.wait(rid)
+            assert lat >= 0
+            assert np.array_equal(got["pattern_id"], expect[i]["pattern_id"]), i
+            assert np.array_equal(got["offset"], expect[i]["offset"]), i
+        # staggered, from two submitting threads, waits in another order
+        ids = [None] * len(reqs)
+
+        def submit_range(lo, hi):
+            for j in range(lo, hi):
+                ids[j] = srv.submit(*reqs[j])
+
+        th = [threading.Thread(target=submit_range, args=(0, 40)), threading.Thread(target=submit_range, args=(40, 80))]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        for i in reversed(range(len(reqs))):
+            got, _ = srv.wait(ids[i])
+            assert np.array_equal(got["pattern_id"], expect[i]["pattern_id"]), i
+            assert np.array_equal(got["offset"], expect[i]["offset"]), i
+        st = srv.stats()
+        assert st["completed"] == 2 * len(reqs) and st["pending"] == 0 and st["in_flight_bytes"] == 0
+        assert st["batches"] < st["completed"]  # requests were batched together
+    for f in filters:
+        f[0].close()
+
+
+def test_server_admission_limits():
+    say("server admission")
+    sbox = fa.sbox_from_seed(92)
+    filters = make_filters(sbox)[:2]
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=4 << 20, slots=2, max_batch_requests=3) as srv:
+        big = np.zeros(3 << 20, dtype=np.uint8)  # larger than one slot (2 MiB)
+        with pytest.raises(fa.FFBFError) as e:
+            srv.submit(0, big)
+        assert e.value.status == fa.ECAPACITY
+        with pytest.raises(fa.FFBFError):
+            srv.submit(5, b"x" * 100)  # no such filter
+        # many requests that only fit a few at a time: all complete, in budget
+        reqs = make_requests(filters, 30, 5, 900_000)
+        expect = [alone(filters, f, t) for f, t in reqs]
+        ids = [srv.submit(f, t) for f, t in reqs]
+        for i, rid in enumerate(ids):
+            got, _ = srv.wait(rid)
+            assert np.array_equal(got["offset"], expect[i]["offset"]), i
+        assert srv.stats()["rejected"] == 1
+    for f in filters:
+        f[0].close()
