@@ -199,6 +199,143 @@ def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0):
     }
 
 
+def serve_inputs(fa, n_requests, seed=51):
+    """Serving workload: 8 resident pattern sets, n log-spaced 10k..2M
+    (lengths 24-80; sets alternate DNA and printable ASCII), W=24, k=3,
+    filter bits = 16 n rounded up to a power of two (the multi-filter rule);
+    requests with log-normal sizes (median 512 KiB, sigma 1.5, clipped to
+    1 KiB-256 MiB), each routed to a seeded set and drawn over that set's
+    alphabet, one planted occurrence per started MiB."""
+    import numpy as np
+
+    W, k = 24, 3
+    sbox = fa.sbox_from_seed(seed)
+    sets = []
+    for f in range(8):
+        n = int(round(10_000 * 200.0 ** (f / 7.0)))
+        alpha = fa.ALPHA_DNA_GC40 if f % 2 == 0 else fa.ALPHA_ASCII
+        pb, po = fa.gen_patterns(alpha, seed * 100 + f, n, 24, 80)
+        sets.append((pb, po, next_pow2(16 * n).bit_length() - 1, alpha))
+    sizes = fa.lognormal_sizes(seed, n_requests, 512 * 1024.0, 1.5, 1024, 256 << 20)
+    route = [int(fa.gen_u64(seed, 23, i) % 8) for i in range(n_requests)]
+    reqs = []
+    for i in range(n_requests):
+        pb, po, _, alpha = sets[route[i]]
+        t = fa.gen_bytes(alpha, seed * 1000 + 1 + route[i], int(sizes[i]), start=int(fa.gen_u64(seed, 24, i) >> 24))
+        fa.plant(seed * 1000 + 9 + i, (t.size + (1 << 20) - 1) >> 20, t, pb, po)
+        reqs.append((route[i], t))
+    # Poisson arrivals: exponential gaps from 53-bit uniforms (log of a
+    # uniform through numpy; the gaps only pace submission)
+    u = np.array([(fa.gen_u64(seed, 25, i) >> 11) / float(1 << 53) for i in range(n_requests)])
+    gaps = -np.log1p(-u)  # mean 1, scaled by 1/lambda at run time
+    return {"W": W, "k": k, "sbox": sbox, "sets": sets, "requests": reqs, "gaps": gaps}
+
+
+def percentile(xs, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def run_serving(fa, n_requests=600, load=0.7, budget=2 << 30):
+    """Continuous-batching server vs the plain per-request path (one
+    ffbf_match_text per request, FIFO) on the same requests: saturation
+    throughput of each, then latency at Poisson arrivals at `load` x the
+    server's saturation rate."""
+    import threading
+
+    import numpy as np
+
+    t0 = time.perf_counter()
+    d = serve_inputs(fa, n_requests)
+    total = sum(t.size for _, t in d["requests"])
+    log(f"serve: {n_requests} requests, {total / 2**30:.2f} GiB, 8 filters, generated in {time.perf_counter() - t0:.1f} s")
+    ms = []
+    for pb, po, log2, _ in d["sets"]:
+        m = fa.Matcher(d["W"], d["k"], log2, d["sbox"])
+        m.set_patterns(pb, po)
+        m.build()
+        ms.append(m)
+    reqs = d["requests"]
+    # plain per-request path: service time of each request alone
+    for f, t in reqs[:8]:
+        ms[f].match(t)  # warm
+    service = []
+    plain_res = []
+    for f, t in reqs:
+        t1 = time.perf_counter()
+        plain_res.append(ms[f].match(t).matches)
+        service.append(time.perf_counter() - t1)
+    plain_s = float(sum(service))
+    srv = fa.Server(ms, hbm_budget_bytes=budget, slots=2)
+    for f, t in reqs[:8]:
+        srv.wait(srv.submit(f, t))  # warm (graph capture)
+    # saturation: everything submitted at once
+    t1 = time.perf_counter()
+    ids = [srv.submit(f, t) for f, t in reqs]
+    same = True
+    for i, rid in enumerate(ids):
+        got, _ = srv.wait(rid)
+        same = same and np.array_equal(got["pattern_id"], plain_res[i]["pattern_id"]) and \
+            np.array_equal(got["offset"], plain_res[i]["offset"])
+    sat_s = time.perf_counter() - t1
+    # Poisson arrivals at load x the saturation request rate
+    lam = load * n_requests / sat_s
+    arrivals = np.cumsum(d["gaps"]) / lam
+    lat = [None] * n_requests
+    rids = [None] * n_requests
+    ready = [threading.Event() for _ in range(n_requests)]
+
+    def submitter():
+        start = time.perf_counter()
+        for i, (f, t) in enumerate(reqs):
+            dt = arrivals[i] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(dt)
+            rids[i] = srv.submit(f, t)
+            ready[i].set()
+
+    th = threading.Thread(target=submitter)
+    t1 = time.perf_counter()
+    th.start()
+    for i in range(n_requests):
+        ready[i].wait()
+        got, lat[i] = srv.wait(rids[i])
+    th.join()
+    poisson_s = time.perf_counter() - t1
+    stats = srv.stats()
+    srv.close()
+    # the plain path under the same arrivals: a FIFO single server with the
+    # measured service times
+    plain_lat, free_at = [], 0.0
+    for i in range(n_requests):
+        start = max(arrivals[i], free_at)
+        free_at = start + service[i]
+        plain_lat.append((free_at - arrivals[i]) * 1e3)
+    for m in ms:
+        m.close()
+    return {
+        "workload": "serve5",
+        "requests": n_requests,
+        "text_bytes": int(total),
+        "filters": 8,
+        "saturation_gbs": round(total / sat_s / 1e9, 3),
+        "plain_gbs": round(total / plain_s / 1e9, 3),
+        "speedup_vs_plain": round(plain_s / sat_s, 2),
+        "same_matches_as_plain": bool(same),
+        "poisson_rate_per_s": round(lam, 1),
+        "poisson_load": load,
+        "p50_ms": round(percentile(lat, 50), 3),
+        "p99_ms": round(percentile(lat, 99), 3),
+        "poisson_gbs": round(total / poisson_s / 1e9, 3),
+        "plain_p50_ms": round(percentile(plain_lat, 50), 3),
+        "plain_p99_ms": round(percentile(plain_lat, 99), 3),
+        "batches": int(stats["batches"]),
+        "hbm_budget_bytes": budget,
+        "slots": 2,
+    }
+
+
 def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
     """Oracle (C restatement of the reference path) on a bounded text sample."""
     import numpy as np
@@ -271,7 +408,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS) + ["batch3", "batch4"])
+    ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS) + ["batch3", "batch4", "serve5"])
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg of the default run")
     ap.add_argument("--mode", type=int, default=-1, help="scan mode override (-1 auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -298,6 +435,15 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
         return 0
 
+    if args.workload == "serve5":
+        r = run_serving(fa, n_requests=max(64, args.steps * 100))
+        print(json.dumps({"metric": METRIC, "value": r["saturation_gbs"], "unit": "GB/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+                          "data": "synthetic (seeded counter-based generator)",
+                          "config": {"workload": "serve5", "requests": r["requests"], "text_bytes": r["text_bytes"]},
+                          "serving": r}), flush=True)
+        return 0
     if args.workload in ("batch3", "batch4"):
         r = run_batched(fa, args.workload, steps=args.steps)
         print(json.dumps({"metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -383,6 +529,8 @@ def main() -> int:
     if args.workload == "plain" and not args.no_batched:
         log("batched leg (batch3) ...")
         out["batched"] = run_batched(fa, "batch3", steps=10)
+        log("serving leg (serve5) ...")
+        out["serving"] = run_serving(fa, n_requests=600)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, sbox, pb, po, text)

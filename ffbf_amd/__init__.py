@@ -134,6 +134,8 @@ ABI = {
     "ffbf_device_download": (_i32, [_P, _P, _u64]),
     "ffbf_device_count": (_i32, [_c.POINTER(_i32)]),
     "ffbf_device_copy": (_i32, [_P, _P, _u64]),
+    "ffbf_host_alloc": (_i32, [_u64, _c.POINTER(_P)]),
+    "ffbf_host_free": (_i32, [_P]),
     "ffbf_batch_create": (_i32, [_P, _u32, _c.POINTER(_P)]),
     "ffbf_batch_destroy": (None, [_P]),
     "ffbf_batch_set_texts": (_i32, [_P, _P, _P, _P, _u32]),
@@ -289,6 +291,30 @@ class DeviceBuffer:
     def free(self) -> None:
         if self.ptr:
             _lib.ffbf_device_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedBuffer:
+    """Page-locked host bytes with a numpy view (uploads at full link speed)."""
+
+    def __init__(self, nbytes: int):
+        p = _c.c_void_p(0)
+        _check(_lib.ffbf_host_alloc(nbytes, _c.byref(p)))
+        self.ptr = p.value or 0
+        self.nbytes = nbytes
+        raw = (_c.c_uint8 * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(raw, dtype=np.uint8)[:nbytes]
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            _lib.ffbf_host_free(self.ptr)
             self.ptr = 0
 
     def __del__(self):

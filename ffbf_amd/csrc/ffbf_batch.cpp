@@ -58,6 +58,10 @@ struct ffbf_batch {
     std::vector<BatchMatchRec> sorted;
 };
 
+namespace ffbf {
+hipStream_t batch_stream(ffbf_batch *b) { return b->stream; }
+}  // namespace ffbf
+
 namespace {
 
 void drop_graph(ffbf_batch *b)
@@ -370,11 +374,17 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
             chunks.push_back(c);
         }
     }
-    const bool grow = tx.size() > b->d_texts.cap || chunks.size() > b->d_chunks.cap;
-    if (grow) drop_graph(b);
-    RET_IF(b->d_texts.ensure(std::max<uint64_t>(tx.size() + tx.size() / 4, 64), "batch text descriptors"));
-    RET_IF(b->d_chunks.ensure(std::max<uint64_t>(chunks.size() + chunks.size() / 4, 256), "batch chunks"));
-    if (b->texts != dev_texts) drop_graph(b);
+    // the captured graph holds these pointers: drop it whenever one changes
+    // (a replay must never read a freed descriptor buffer)
+    const BatchText *old_texts = b->d_texts.p;
+    const BatchChunk *old_chunks = b->d_chunks.p;
+    if (tx.size() > b->d_texts.cap)
+        RET_IF(b->d_texts.ensure(std::max<uint64_t>(tx.size() + tx.size() / 4, 64), "batch text descriptors"));
+    if (chunks.size() > b->d_chunks.cap)
+        RET_IF(b->d_chunks.ensure(std::max<uint64_t>(chunks.size() + chunks.size() / 4, 256), "batch chunks"));
+    if (!b->d_texts.p) RET_IF(b->d_texts.ensure(64, "batch text descriptors"));
+    if (!b->d_chunks.p) RET_IF(b->d_chunks.ensure(256, "batch chunks"));
+    if (b->d_texts.p != old_texts || b->d_chunks.p != old_chunks || b->texts != dev_texts) drop_graph(b);
     b->texts = dev_texts;
     b->n_texts = n_texts;
     b->n_chunks = (uint32_t)chunks.size();

@@ -807,6 +807,25 @@ int ffbf_device_copy(void *dst, const void *src, uint64_t bytes)
     return FFBF_OK;
 }
 
+int ffbf_host_alloc(uint64_t bytes, void **out)
+{
+    if (!out) return fail(FFBF_EINVAL, "null output");
+    *out = nullptr;
+    hipError_t e = hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(FFBF_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    return FFBF_OK;
+}
+
+int ffbf_host_free(void *ptr)
+{
+    if (ptr) HIP_TRY(hipHostFree(ptr));
+    return FFBF_OK;
+}
+
 int ffbf_device_count(int *out)
 {
     if (!out) return fail(FFBF_EINVAL, "null output");

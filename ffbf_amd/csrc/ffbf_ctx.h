@@ -113,6 +113,8 @@ struct ffbf_ctx {
 };
 
 namespace ffbf {
+// The stream a batch object queues its work on (ffbf_batch.cpp).
+hipStream_t batch_stream(ffbf_batch *b);
 // Stage launchers shared with the batch path (ffbf_capi.cpp).
 int enqueue_pass(ffbf_ctx *c);
 int use_device(ffbf_ctx *c);

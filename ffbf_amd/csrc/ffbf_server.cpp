@@ -44,7 +44,6 @@ struct Slot {
     std::vector<ffbf_ctx *> filters;  // owned replicas
     ffbf_batch *batch = nullptr;
     uint8_t *d_text = nullptr;
-    uint8_t *h_stage = nullptr;
     uint64_t cap = 0;
     std::thread th;
 };
@@ -65,8 +64,11 @@ int clone_ctx(const ffbf_ctx *src, ffbf_ctx **out)
     std::vector<uint8_t> bytes(off.back());
     if (!bytes.empty()) HIP_TRY(hipMemcpy(bytes.data(), src->bytes.p, bytes.size(), hipMemcpyDeviceToHost));
     RET_IF(ffbf_set_patterns(c, bytes.data(), off.data(), src->n_pat));
-    // F as built (or loaded) by the source, then the same mode choice
-    HIP_TRY(hipMemcpy(c->F.p, src->F.p, src->f_words * 4, hipMemcpyDeviceToDevice));
+    // F as built (or loaded) by the source, then the same mode choice.  The
+    // copy is ordered on the clone's stream before the folds computed from
+    // it (a device-to-device hipMemcpy may return before it completes).
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    HIP_TRY(hipMemcpyAsync(c->F.p, src->F.p, src->f_words * 4, hipMemcpyDeviceToDevice, c->stream));
     c->f_set_bits = src->f_set_bits;
     c->built = true;
     c->mode_req = src->mode_req;
@@ -125,21 +127,24 @@ void serve_slot(ffbf_server *s, Slot *slot)
                 continue;
             }
         }
-        // stage the texts back to back and scan them as one batch
+        // copy the texts back to back into the slot's device buffer, on the
+        // batch's stream (straight from the caller's memory: pinned memory
+        // goes at full link speed), then scan them as one batch
         offsets.assign(work.size() + 1, 0);
         route.resize(work.size());
-        for (size_t i = 0; i < work.size(); i++) {
-            std::memcpy(slot->h_stage + offsets[i], work[i]->text, work[i]->n);
-            offsets[i + 1] = offsets[i] + work[i]->n;
-            route[i] = work[i]->filter;
-        }
         int st = FFBF_OK;
         std::string err;
         uint64_t nm = 0;
-        if (offsets.back() &&
-            hipMemcpy(slot->d_text, slot->h_stage, offsets.back(), hipMemcpyHostToDevice) != hipSuccess) {
-            st = FFBF_EDEVICE;
-            err = std::string("text upload: ") + hipGetErrorString(hipGetLastError());
+        const hipStream_t stream = batch_stream(slot->batch);
+        for (size_t i = 0; i < work.size(); i++) {
+            if (work[i]->n && st == FFBF_OK &&
+                hipMemcpyAsync(slot->d_text + offsets[i], work[i]->text, work[i]->n, hipMemcpyHostToDevice, stream) !=
+                    hipSuccess) {
+                st = FFBF_EDEVICE;
+                err = std::string("text upload: ") + hipGetErrorString(hipGetLastError());
+            }
+            offsets[i + 1] = offsets[i] + work[i]->n;
+            route[i] = work[i]->filter;
         }
         if (st == FFBF_OK) st = ffbf_batch_set_texts(slot->batch, slot->d_text, offsets.data(), route.data(), (uint32_t)work.size());
         if (st == FFBF_OK) st = ffbf_batch_run(slot->batch, &nm);
@@ -189,7 +194,6 @@ void shutdown(ffbf_server *s)
         if (sl.batch) ffbf_batch_destroy(sl.batch);
         for (ffbf_ctx *c : sl.filters) ffbf_destroy(c);
         if (sl.d_text) (void)hipFree(sl.d_text);
-        if (sl.h_stage) (void)hipHostFree(sl.h_stage);
     }
     for (auto &kv : s->reqs) delete kv.second;
     s->reqs.clear();
@@ -224,8 +228,7 @@ int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_
         if (st == FFBF_OK) st = ffbf_batch_create(sl.filters.data(), n_filters, &sl.batch);
         if (st == FFBF_OK) {
             sl.cap = s->cfg.max_batch_bytes;
-            if (hipMalloc((void **)&sl.d_text, std::max<uint64_t>(sl.cap, 16)) != hipSuccess ||
-                hipHostMalloc((void **)&sl.h_stage, std::max<uint64_t>(sl.cap, 16), hipHostMallocDefault) != hipSuccess) {
+            if (hipMalloc((void **)&sl.d_text, std::max<uint64_t>(sl.cap, 16)) != hipSuccess) {
                 (void)hipGetLastError();
                 st = fail(FFBF_ENOMEM, "server slot buffers");
             }

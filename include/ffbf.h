@@ -261,6 +261,9 @@ int ffbf_device_free(void *ptr);
 int ffbf_device_upload(void *dst, const void *src, uint64_t bytes);
 int ffbf_device_download(void *dst, const void *src, uint64_t bytes);
 int ffbf_device_count(int *out);
+/* Page-locked host memory (texts submitted from it upload at full link speed). */
+int ffbf_host_alloc(uint64_t bytes, void **out);
+int ffbf_host_free(void *ptr);
 /* Plain copy kernel over 16-byte aligned device buffers (synchronous); the
  * reference point for HBM byte counters. */
 int ffbf_device_copy(void *dst, const void *src, uint64_t bytes);

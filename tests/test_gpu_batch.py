@@ -174,3 +174,41 @@ def test_batch_candidate_overflow():
     dev.free()
     for f in filters:
         f[0].close()
+
+
+def test_batch_regrows_descriptor_buffers_between_batches():
+    # chunk counts that cross the descriptor buffers' capacity: every new
+    # batch must be scanned from its own descriptors (the replayed graph is
+    # rebuilt whenever a buffer moves)
+    say("batch descriptor regrowth")
+    sbox = fa.sbox_from_seed(81)
+    filters = make_filters(sbox, SPECS[:2], 700)
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_tuning(chunk_iters=32)
+        for round_, n_chunks in enumerate([10, 230, 250, 270, 300, 330, 500, 20]):
+            # texts of 4..12 chunks of 32 KiB each, up to n_chunks in total
+            rng = np.random.default_rng(round_)
+            lens, left = [], n_chunks
+            while left > 0:
+                c = int(min(left, rng.integers(4, 13)))
+                lens.append(c * 32 * 1024 - int(rng.integers(0, 100)))
+                left -= c
+            route = (np.arange(len(lens)) % len(filters)).astype(np.uint32)
+            texts = []
+            for i, n in enumerate(lens):
+                f = int(route[i])
+                t = fa.gen_bytes(fa.ALPHA_DNA_GC40 if f % 2 == 0 else fa.ALPHA_ASCII, 50 * round_ + i, n)
+                fa.plant(70 * round_ + i, 3, t, filters[f][1], filters[f][2])
+                texts.append(t)
+            off = np.zeros(len(lens) + 1, dtype=np.uint64)
+            off[1:] = np.cumsum(lens)
+            dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+            bt.set_texts(dev.ptr, off, route)
+            bt.run()
+            got = as_rows(bt.matches())
+            bt.run_per_text()
+            assert got == as_rows(bt.matches()), f"round {round_} ({n_chunks} chunks)"
+            assert len(got) >= len(lens)
+            dev.free()
+    for f in filters:
+        f[0].close()

@@ -21,7 +21,10 @@ def say(*a):
 
 
 def make_filters(sbox):
-    specs = [(800, 14, fa.ALPHA_DNA_GC40), (5000, 17, fa.ALPHA_ASCII), (40000, 20, fa.ALPHA_DNA_GC40)]
+    # whole filter in LDS, LDS fold, F in L2, L2-sized global fold: each mode
+    # is copied into the server's replicas differently
+    specs = [(800, 14, fa.ALPHA_DNA_GC40), (5000, 22, fa.ALPHA_ASCII), (40000, 20, fa.ALPHA_DNA_GC40),
+             (200000, 22, fa.ALPHA_DNA_GC40), (1_000_000, 26, fa.ALPHA_ASCII)]
     out = []
     for i, (n, log2, alpha) in enumerate(specs):
         b, off = fa.gen_patterns(alpha, 900 + i, n, W, 70)
@@ -54,6 +57,7 @@ def test_server_results_equal_alone():
     say("server mix vs alone")
     sbox = fa.sbox_from_seed(91)
     filters = make_filters(sbox)
+    say(f"  filter modes {[f[0].timings()['scan_mode'] for f in filters]}")
     reqs = make_requests(filters, 80, 3, 300_000)
     expect = [alone(filters, f, t) for f, t in reqs]
     # spot-check the reference answer itself against the oracle
@@ -90,6 +94,29 @@ def test_server_results_equal_alone():
         st = srv.stats()
         assert st["completed"] == 2 * len(reqs) and st["pending"] == 0 and st["in_flight_bytes"] == 0
         assert st["batches"] < st["completed"]  # requests were batched together
+    for f in filters:
+        f[0].close()
+
+
+def test_server_pinned_requests():
+    say("server pinned requests")
+    sbox = fa.sbox_from_seed(93)
+    filters = make_filters(sbox)
+    reqs = make_requests(filters, 40, 4, 400_000)
+    pins = []
+    for f, t in reqs:
+        p = fa.PinnedBuffer(t.size)
+        p.array[:] = t
+        pins.append((f, p))
+    expect = [alone(filters, f, t) for f, t in reqs]
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=32 << 20, slots=2) as srv:
+        ids = [srv.submit(f, p.array) for f, p in pins]
+        for i, rid in enumerate(ids):
+            got, _ = srv.wait(rid)
+            assert np.array_equal(got["pattern_id"], expect[i]["pattern_id"]), i
+            assert np.array_equal(got["offset"], expect[i]["offset"]), i
+    for _, p in pins:
+        p.free()
     for f in filters:
         f[0].close()
 
