@@ -149,6 +149,7 @@ ABI = {
     "ffbf_server_create": (_i32, [_P, _u32, _P, _c.POINTER(_P)]),
     "ffbf_server_destroy": (None, [_P]),
     "ffbf_server_submit": (_i32, [_P, _u32, _P, _u64, _pu64]),
+    "ffbf_server_submit_many": (_i32, [_P, _u32, _P, _P, _P, _P]),
     "ffbf_server_poll": (_i32, [_P, _u64, _c.POINTER(_c.c_int)]),
     "ffbf_server_wait": (_i32, [_P, _u64, _c.POINTER(_P), _pu64, _c.POINTER(_c.c_double)]),
     "ffbf_server_get_stats": (_i32, [_P, _P]),
@@ -607,6 +608,20 @@ class Server:
         _check(_lib.ffbf_server_submit(self._h, filter_id, _ptr(t), t.size, _c.byref(rid)))
         self._keep[rid.value] = t  # the text must outlive the request
         return rid.value
+
+    def submit_many(self, requests) -> list:
+        """requests: iterable of (filter id, uint8 array); one C call."""
+        reqs = [(f, np.frombuffer(t, dtype=np.uint8) if isinstance(t, (bytes, bytearray)) else np.ascontiguousarray(t))
+                for f, t in requests]
+        n = len(reqs)
+        fil = np.array([f for f, _ in reqs], dtype=np.uint32)
+        ptrs = np.array([_ptr(t) for _, t in reqs], dtype=np.uint64)
+        sizes = np.array([t.size for _, t in reqs], dtype=np.uint64)
+        ids = np.zeros(n, dtype=np.uint64)
+        _check(_lib.ffbf_server_submit_many(self._h, n, _ptr(fil), _ptr(ptrs), _ptr(sizes), _ptr(ids)))
+        for rid, (_, t) in zip(ids.tolist(), reqs):
+            self._keep[rid] = t
+        return ids.tolist()
 
     def poll(self, rid: int) -> bool:
         d = _c.c_int(0)

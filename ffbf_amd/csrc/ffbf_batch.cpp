@@ -239,8 +239,11 @@ int sort_on_device(ffbf_batch *b)
     const int end_bit = (int)(bo + bi + bt);
     size_t tb = 0;
     HIP_TRY(sort_temp_bytes(b->n_match, end_bit, &tb));
-    RET_IF(b->sort_keys.ensure(2 * b->n_match, "batch sort keys"));
-    RET_IF(b->sort_temp.ensure(tb, "batch sort scratch"));
+    // geometric growth: a reallocation (hipFree) stalls the whole device
+    if (2 * b->n_match > b->sort_keys.cap)
+        RET_IF(b->sort_keys.ensure(std::max<uint64_t>(4 * b->n_match, 1 << 17), "batch sort keys"));
+    if (tb > b->sort_temp.cap) RET_IF(b->sort_temp.ensure(std::max<uint64_t>(2 * tb, 1 << 20), "batch sort scratch"));
+    tb = b->sort_temp.cap;
     HIP_TRY(sort_batch_matches(b->matches.p, b->n_match, b->sort_keys.p, b->sort_keys.p + b->n_match, b->sort_temp.p, tb,
                                (int)bo, (int)(bo + bi), end_bit, b->stream));
     b->sorted_on_device = true;
@@ -378,12 +381,14 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     // (a replay must never read a freed descriptor buffer)
     const BatchText *old_texts = b->d_texts.p;
     const BatchChunk *old_chunks = b->d_chunks.p;
+    // grow geometrically so that a stream of batches settles quickly (every
+    // move costs a re-capture)
     if (tx.size() > b->d_texts.cap)
-        RET_IF(b->d_texts.ensure(std::max<uint64_t>(tx.size() + tx.size() / 4, 64), "batch text descriptors"));
+        RET_IF(b->d_texts.ensure(std::max<uint64_t>(2 * tx.size(), 1024), "batch text descriptors"));
     if (chunks.size() > b->d_chunks.cap)
-        RET_IF(b->d_chunks.ensure(std::max<uint64_t>(chunks.size() + chunks.size() / 4, 256), "batch chunks"));
-    if (!b->d_texts.p) RET_IF(b->d_texts.ensure(64, "batch text descriptors"));
-    if (!b->d_chunks.p) RET_IF(b->d_chunks.ensure(256, "batch chunks"));
+        RET_IF(b->d_chunks.ensure(std::max<uint64_t>(2 * chunks.size(), 4096), "batch chunks"));
+    if (!b->d_texts.p) RET_IF(b->d_texts.ensure(1024, "batch text descriptors"));
+    if (!b->d_chunks.p) RET_IF(b->d_chunks.ensure(4096, "batch chunks"));
     if (b->d_texts.p != old_texts || b->d_chunks.p != old_chunks || b->texts != dev_texts) drop_graph(b);
     b->texts = dev_texts;
     b->n_texts = n_texts;
@@ -394,7 +399,7 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     const uint64_t want = std::max<uint64_t>(1 << 16, total >> 10);
     if (b->cand.cap < want) {
         drop_graph(b);
-        RET_IF(b->cand.ensure(want, "batch candidates"));
+        RET_IF(b->cand.ensure(std::max(want, 2 * b->cand.cap), "batch candidates"));
         RET_IF(refresh_filters(b));
     }
     if (n_texts) HIP_TRY(hipMemcpyAsync(b->d_texts.p, tx.data(), tx.size() * sizeof(BatchText), hipMemcpyHostToDevice, b->stream));
@@ -419,8 +424,8 @@ int ffbf_batch_run(ffbf_batch *b, uint64_t *n_matches)
         if (nc <= b->cand.cap && nm <= b->matches.cap) break;
         // grow and run again: every stage is a pure function of its inputs
         drop_graph(b);
-        if (nc > b->cand.cap) RET_IF(b->cand.ensure(nc + nc / 4 + 1024, "batch candidates"));
-        if (nm > b->matches.cap) RET_IF(b->matches.ensure(nm + nm / 4 + 1024, "batch matches"));
+        if (nc > b->cand.cap) RET_IF(b->cand.ensure(std::max(2 * nc, 2 * b->cand.cap), "batch candidates"));
+        if (nm > b->matches.cap) RET_IF(b->matches.ensure(std::max(2 * nm, 2 * b->matches.cap), "batch matches"));
         RET_IF(refresh_filters(b));
         if (attempt == 2) return fail(FFBF_EDEVICE, "batch outputs kept overflowing");
     }

@@ -164,7 +164,9 @@ int choose_mode(ffbf_ctx *c)
     if (c->probes_req) {
         c->probes = std::min(c->probes_req, k);
     } else if (mode == ffbf::MODE_FOLD || mode == ffbf::MODE_LDS) {
-        c->probes = c->pre_fill < 0.15 ? 1u : 2u;
+        // measured on 64 MiB-1.5 GiB texts: fill 0.04 (10k patterns, 1 Mbit
+        // fold) is fastest at 2, fill 0.32 (100k patterns) at 3
+        c->probes = c->pre_fill < 0.01 ? 1u : (c->pre_fill < 0.2 ? 2u : 3u);
         c->probes = std::min(c->probes, k);
     } else {
         c->probes = 1;
@@ -273,7 +275,7 @@ int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n)
     c->text = dev_text;
     c->text_n = n;
     const uint64_t want = std::max<uint64_t>(65536, n >> 10);
-    if (c->cand.cap < want) RET_IF(c->cand.ensure(want, "candidates"));
+    if (c->cand.cap < want) RET_IF(c->cand.ensure(std::max(want, 2 * c->cand.cap), "candidates"));
     if (!c->matches.p) RET_IF(c->matches.ensure(65536, "matches"));
     c->scanned = c->passed = c->verified = false;
     c->sorted_valid = false;
@@ -286,7 +288,7 @@ int do_scan(ffbf_ctx *c)
     RET_IF(enqueue_scan(c));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
-        RET_IF(c->cand.ensure(c->h_counters[ffbf::C_CAND] + c->h_counters[ffbf::C_CAND] / 4 + 1024, "candidates"));
+        RET_IF(c->cand.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_CAND], 2 * c->cand.cap), "candidates"));
         RET_IF(enqueue_scan(c));
         RET_IF(sync_counters(c));
     }
@@ -305,8 +307,10 @@ int sort_on_device(ffbf_ctx *c)
     const int end_bit = ffbf::sort_end_bit(c->n_pat);
     size_t tb = 0;
     HIP_TRY(ffbf::sort_temp_bytes(c->n_match, end_bit, &tb));
-    RET_IF(c->sort_keys.ensure(2 * c->n_match, "sort keys"));
-    RET_IF(c->sort_temp.ensure(tb, "sort scratch"));
+    if (2 * c->n_match > c->sort_keys.cap)
+        RET_IF(c->sort_keys.ensure(std::max<uint64_t>(4 * c->n_match, 1 << 17), "sort keys"));
+    if (tb > c->sort_temp.cap) RET_IF(c->sort_temp.ensure(std::max<uint64_t>(2 * tb, 1 << 20), "sort scratch"));
+    tb = c->sort_temp.cap;
     HIP_TRY(ffbf::sort_matches(c->matches.p, c->n_match, c->sort_keys.p, c->sort_keys.p + c->n_match, c->sort_temp.p,
                                tb, end_bit, c->stream));
     c->sorted_on_device = true;
@@ -318,7 +322,7 @@ int do_verify(ffbf_ctx *c)
     RET_IF(enqueue_verify(c));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
-        RET_IF(c->matches.ensure(c->h_counters[ffbf::C_MATCH] + c->h_counters[ffbf::C_MATCH] / 4 + 1024, "matches"));
+        RET_IF(c->matches.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_MATCH], 2 * c->matches.cap), "matches"));
         RET_IF(enqueue_verify(c));
         RET_IF(sync_counters(c));
     }
@@ -592,7 +596,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
         // rerun the three stages with room for every candidate
-        RET_IF(c->cand.ensure(c->h_counters[ffbf::C_CAND] + c->h_counters[ffbf::C_CAND] / 4 + 1024, "candidates"));
+        RET_IF(c->cand.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_CAND], 2 * c->cand.cap), "candidates"));
         RET_IF(enqueue_scan(c));
         RET_IF(enqueue_pass(c));
         RET_IF(enqueue_verify(c));
@@ -602,7 +606,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     c->n_surv = c->h_counters[ffbf::C_SURV];
     c->scanned = c->passed = true;
     if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
-        RET_IF(c->matches.ensure(c->h_counters[ffbf::C_MATCH] + c->h_counters[ffbf::C_MATCH] / 4 + 1024, "matches"));
+        RET_IF(c->matches.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_MATCH], 2 * c->matches.cap), "matches"));
         RET_IF(enqueue_verify(c));
         RET_IF(sync_counters(c));
     }

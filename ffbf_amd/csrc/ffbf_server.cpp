@@ -285,6 +285,14 @@ int ffbf_server_submit(ffbf_server *s, uint32_t filter, const uint8_t *text, uin
     return FFBF_OK;
 }
 
+int ffbf_server_submit_many(ffbf_server *s, uint32_t n, const uint32_t *filters, const uint8_t *const *texts,
+                            const uint64_t *sizes, uint64_t *request_ids)
+{
+    if (!s || (n && (!filters || !texts || !sizes || !request_ids))) return fail(FFBF_EINVAL, "null argument");
+    for (uint32_t i = 0; i < n; i++) RET_IF(ffbf_server_submit(s, filters[i], texts[i], sizes[i], &request_ids[i]));
+    return FFBF_OK;
+}
+
 int ffbf_server_poll(ffbf_server *s, uint64_t request_id, int *done)
 {
     if (!s || !done) return fail(FFBF_EINVAL, "null argument");

@@ -248,6 +248,9 @@ void ffbf_server_destroy(ffbf_server *server);
  * completes) against filter `filter`.  FFBF_ECAPACITY if the request can
  * never be admitted (larger than a slot or the budget). */
 int ffbf_server_submit(ffbf_server *server, uint32_t filter, const uint8_t *text, uint64_t n, uint64_t *request_id);
+/* ffbf_server_submit for n requests at once (stops at the first error). */
+int ffbf_server_submit_many(ffbf_server *server, uint32_t n, const uint32_t *filters, const uint8_t *const *texts,
+                            const uint64_t *sizes, uint64_t *request_ids);
 int ffbf_server_poll(ffbf_server *server, uint64_t request_id, int *done);
 /* Blocks until the request completes; *matches (sorted by pattern id, then
  * offset) is allocated by the library (ffbf_free_matches); latency_ms is
