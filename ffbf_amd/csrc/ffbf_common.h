@@ -18,6 +18,15 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t rotl(uint32_t x, uint32_t r) { return __builtin_rotateleft32(x, r); }
 
+// a ^ b ^ c in one VALU op (gfx950 three-input bitwise op, truth table 0x96);
+// the compiler does not form it from XOR chains by itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t fwd_idx(int j, uint32_t h1, uint32_t h2)
 {
     switch (j) {
@@ -43,11 +52,12 @@ __device__ __forceinline__ uint32_t rev_idx(int j, uint32_t h1, uint32_t h2)
 __device__ __forceinline__ uint32_t bit_of(const uint32_t *v, uint32_t b) { return (v[b >> 5] >> (b & 31u)) & 1u; }
 
 // Bit x (taken modulo 32*2^log2_words) of a filter held in LDS: word index by
-// one bit-field extract, bit by another (its offset operand uses x's low 5 bits).
+// one bit-field extract, bit by another (v_bfe_u32 reads only the low 5 bits
+// of its offset operand, so x needs no masking).
 __device__ __forceinline__ uint32_t lds_bit(const uint32_t *s, uint32_t x, uint32_t log2_words)
 {
     const uint32_t w = s[__builtin_amdgcn_ubfe(x, 5u, log2_words)];
-    return __builtin_amdgcn_ubfe(w, x & 31u, 1u);
+    return __builtin_amdgcn_ubfe(w, x, 1u);
 }
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
@@ -74,6 +84,19 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v)
     v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118 /* row_shr:8 */, 0xF, 0xF, true);
     return __builtin_amdgcn_readlane(v, 15) | __builtin_amdgcn_readlane(v, 31) | __builtin_amdgcn_readlane(v, 47) |
            __builtin_amdgcn_readlane(v, 63);
+}
+
+// Sum of v over lanes 0..lane_id() (inclusive): DPP row shifts build the sums
+// inside each 16-lane row, then row broadcasts carry the row totals upward.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111 /* row_shr:1 */, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112 /* row_shr:2 */, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114 /* row_shr:4 */, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118 /* row_shr:8 */, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142 /* row_bcast:15 */, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143 /* row_bcast:31 */, 0xC, 0xF, false);
+    return v;
 }
 
 // Orders this wave's LDS traffic across its lanes: writes before the call are

@@ -14,8 +14,9 @@ namespace ffbf {
 
 constexpr int kUnit = 16;
 
-// Per-wave survivor queue: capacity and the depth at which it is drained.
-// Appends add at most 64 entries, so cap >= drain_at + 63.
+// Per-wave survivor queue: capacity and the depth at which it is drained
+// (64 entries at a time, one per lane).  A slot-by-slot append adds at most 64
+// entries, so cap >= drain_at + 63; a bulk append is taken only when it fits.
 template <int BLOCK>
 struct QueueGeom {
     static constexpr int cap = 128;
@@ -25,6 +26,12 @@ template <>
 struct QueueGeom<1024> {
     static constexpr int cap = 120;
     static constexpr int drain_at = 57;
+};
+// the global-prefilter scan passes ~1 window in 8: room for two drains' worth
+template <>
+struct QueueGeom<256> {
+    static constexpr int cap = 256;
+    static constexpr int drain_at = 64;
 };
 
 // One aligned 16-byte text chunk c (index relative to the 16-byte aligned base
@@ -101,18 +108,24 @@ static __device__ __forceinline__ void hash_chunks(const uint4 (&ch)[Chunks<W>::
         const uint32_t w = d == 0 ? c.x : d == 1 ? c.y : d == 2 ? c.z : c.w;
         s[j] = sb[(w >> (8 * (j & 3))) & 0xFFu];
     }
-    uint32_t x1 = 0, x2 = 0;
+    // first window as a sum of independent rotated terms (pairs fuse into
+    // three-input XORs), the other 15 by rolling
+    uint32_t x1 = s[W - 1], x2 = s[W - 1];
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-        x1 = rotl(x1, 1) ^ s[j];
-        x2 = rotl(x2, 3) ^ s[j];
+    for (int j = 0; j + 1 < W - 1; j += 2) {
+        x1 = xor3(x1, rotl(s[j], (uint32_t)((W - 1 - j) & 31)), rotl(s[j + 1], (uint32_t)((W - 2 - j) & 31)));
+        x2 = xor3(x2, rotl(s[j], (uint32_t)((3 * (W - 1 - j)) & 31)), rotl(s[j + 1], (uint32_t)((3 * (W - 2 - j)) & 31)));
+    }
+    if constexpr (((W - 1) & 1) != 0) {
+        x1 ^= rotl(s[W - 2], 1u);
+        x2 ^= rotl(s[W - 2], 3u);
     }
     h1[0] = x1;
     h2[0] = x2;
 #pragma unroll
     for (int b = 1; b < kUnit; b++) {
-        x1 = rotl(x1, 1) ^ s[b + W - 1] ^ rotl(s[b - 1], (uint32_t)W);
-        x2 = rotl(x2, 3) ^ s[b + W - 1] ^ rotl(s[b - 1], (uint32_t)(3 * W));
+        x1 = xor3(rotl(x1, 1), s[b + W - 1], rotl(s[b - 1], (uint32_t)W));
+        x2 = xor3(rotl(x2, 3), s[b + W - 1], rotl(s[b - 1], (uint32_t)(3 * W)));
         h1[b] = x1;
         h2[b] = x2;
     }
@@ -243,8 +256,38 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         for (int b = 0; b < kUnit; b++) pm |= ((fw[b] >> (h1[b] & 31u)) & 1u) << b;
     }
     pm &= vm;
-    // visit only the slots where some lane passed; one drain site.  b is
-    // wave-uniform, so h1[b]/h2[b] index registers directly (no scratch)
+    const uint32_t mine = (uint32_t)__builtin_popcount(pm);
+    const uint32_t incl = wave_incl_sum(mine);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    if (total == 0u) return;
+    if (qn + total <= (uint32_t)QG::cap) {
+        // room for every survivor: each lane writes its own at its rank in
+        // the wave (exclusive prefix of the per-lane counts), no ballots
+        uint4 *dst = q + (qn + incl - mine);
+        // tag + (p0 + b) == tag | (p0 + b) for the real windows (0 <= p0+b <
+        // 2^40); p0 itself may be negative on a text's first unit
+        const uint64_t pt = tag + (uint64_t)p0;
+#pragma unroll
+        for (int b = 0; b < kUnit; b++) {
+            if ((pm >> b) & 1u) {
+                const uint64_t p = pt + (uint64_t)b;
+                *dst = make_uint4((uint32_t)p, (uint32_t)(p >> 32), h1[b], h2[b]);
+                ++dst;
+            }
+        }
+        qn += total;
+        while (qn >= (uint32_t)QG::drain_at) {
+            const uint32_t cnt = qn < 64u ? qn : 64u;
+            wave_sync();
+            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre);
+            wave_sync();
+            qn -= cnt;
+        }
+        return;
+    }
+    // more survivors than queue room: slot by slot, draining as it fills.
+    // Visit only the slots where some lane passed; b is wave-uniform, so
+    // h1[b]/h2[b] index registers directly (no scratch)
     for (uint32_t any = wave_or(pm); any; any &= any - 1u) {
         const uint32_t b = (uint32_t)__builtin_ctz(any);
         const bool pass = (pm >> b) & 1u;

@@ -212,3 +212,41 @@ def test_batch_regrows_descriptor_buffers_between_batches():
             dev.free()
     for f in filters:
         f[0].close()
+
+
+def test_batch_matches_at_unaligned_text_starts():
+    # every text starts at a different residue mod 16 in the batch buffer and
+    # holds a pattern at its very first offsets, inside the 16-byte unit that
+    # straddles the text's start: those candidates must keep their text id
+    say("batch matches at unaligned text starts")
+    sbox = fa.sbox_from_seed(82)
+    filters = make_filters(sbox, SPECS[:3], 800)
+    rng = np.random.default_rng(11)
+    texts, lens = [], []
+    route = (np.arange(48) % len(filters)).astype(np.uint32)
+    for i in range(48):
+        f = int(route[i])
+        _, b, off, _ = filters[f]
+        n = 1001 + 16 * int(rng.integers(0, 40))  # n = 9 mod 16: starts walk all residues
+        t = fa.gen_bytes(fa.ALPHA_DNA_GC40 if f % 2 == 0 else fa.ALPHA_ASCII, 900 + i, n)
+        pid = int(rng.integers(0, off.size - 1))
+        p = b[int(off[pid]):int(off[pid + 1])]
+        at = i % 12
+        t[at:at + p.size] = p
+        texts.append(t)
+        lens.append(n)
+    off = np.zeros(len(lens) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(lens)
+    assert len({int(o) % 16 for o in off[:-1]}) == 16
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    expect, _ = oracle_batch(sbox, filters, texts, route)
+    assert sum(1 for r in expect if r[2] < 12) >= 48
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+        bt.run_per_text()
+        assert as_rows(bt.matches()) == expect
+    dev.free()
+    for f in filters:
+        f[0].close()
