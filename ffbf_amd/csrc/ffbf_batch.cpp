@@ -353,20 +353,21 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
         total += tx[i].n;
         maxn = std::max(maxn, tx[i].n);
     }
-    // chunks of chunk_iters 1 KiB iterations, ordered by filter then text
+    // chunks of chunk_iters wave-iterations (kIterUnits units of 16 bytes),
+    // ordered by filter then text
     std::vector<uint32_t> order(n_texts);
     for (uint32_t i = 0; i < n_texts; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t x, uint32_t y) { return tx[x].filter < tx[y].filter; });
     const uint32_t ci = b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u;
     std::vector<BatchChunk> chunks;
-    chunks.reserve(total / (1024ull * ci) + n_texts + 1);
+    chunks.reserve(total / (16ull * kIterUnits * ci) + n_texts + 1);
     for (uint32_t i : order) {
         const uint64_t lead = (uint64_t)((uintptr_t)(dev_texts + tx[i].start) & 15u);
         const uint64_t npos = tx[i].n >= b->W ? tx[i].n - b->W + 1 : 0;
         if (!npos) continue;
         const uint64_t units = (lead + npos + 15u) >> 4;
-        const uint64_t iters = (units + 63u) >> 6;
+        const uint64_t iters = (units + kIterUnits - 1u) / kIterUnits;
         if (iters > 0xFFFFFFFFull) return fail(FFBF_EINVAL, "text too long for one batch");
         for (uint64_t it = 0; it < iters; it += ci) {
             BatchChunk c;

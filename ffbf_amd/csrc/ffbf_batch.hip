@@ -4,7 +4,7 @@
 // The reference scans one text against one filter per run
 // (greptest.cu:505-576).  Here the texts of a batch sit back to back in one
 // device buffer, each routed to one filter; the host cuts every text into
-// chunks of 1 KiB-iterations, ordered by filter, and persistent workgroups
+// chunks of wave-iterations (62 units of 16 bytes), ordered by filter, and persistent workgroups
 // pull chunks from a global counter.  A workgroup reloads its LDS prefilter
 // only when the next chunk belongs to another filter, and drains its survivor
 // queues before it does.  Candidates carry their text id in the top bits of
@@ -35,7 +35,6 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
     uint32_t loaded = 0xFFFFFFFFu;
     ScanArgs a{};
     bool lds = false;
-    constexpr int NCH = WT > 0 ? Chunks<(WT > 0 ? WT : 1)>::NCH : 1;
 
     for (;;) {
         __syncthreads();
@@ -74,19 +73,18 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         const uint64_t it_end = (uint64_t)c.it0 + c.n_it;
         uint64_t it = (uint64_t)c.it0 + (uint64_t)wv;
 
-        WaveChunks<NCH> cur;
+        uint4 cur = make_uint4(0u, 0u, 0u, 0u);
         if constexpr (WT > 0) {
-            if (it < it_end) cur.load(base, lead, t.n, it << 6);
+            if (it < it_end) cur = load_chunk(base, lead, t.n, it * kIterUnits + (uint64_t)lane_id());
         }
         for (; it < it_end; it += BLOCK / 64) {
-            const uint64_t u = (it << 6) + (uint64_t)lane_id();
+            const uint64_t u = it * kIterUnits + (uint64_t)lane_id();
             uint32_t h1[kUnit], h2[kUnit];
             if constexpr (WT > 0) {
-                WaveChunks<NCH> nxt;
-                if (it + BLOCK / 64 < it_end) nxt.load(base, lead, t.n, (it + BLOCK / 64) << 6);
-                uint4 ch[NCH];
-                cur.spread(ch);
-                hash_chunks<(WT > 0 ? WT : 1)>(ch, s_sbox, h1, h2);
+                uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
+                if (it + BLOCK / 64 < it_end)
+                    nxt = load_chunk(base, lead, t.n, (it + BLOCK / 64) * kIterUnits + (uint64_t)lane_id());
+                hash_unit<(WT > 0 ? WT : 1)>(cur, s_sbox, h1, h2);
                 cur = nxt;
             } else {
                 hash_unit_generic(base, lead, t.n, u, W, s_sbox, h1, h2);

@@ -55,7 +55,7 @@ struct ScanArgs {
 
 // Batched scan: many texts (concatenated in one device buffer), each routed
 // to one of several resident filters, walked by persistent workgroups that
-// pull chunks of 1 KiB-iterations from a global counter.
+// pull chunks of wave-iterations (kIterUnits 16-byte units) from a global counter.
 struct BatchText {
     uint64_t start;  // offset of the text in the batch buffer
     uint64_t n;      // bytes
@@ -66,7 +66,7 @@ struct BatchText {
 struct BatchChunk {
     uint32_t text;
     uint32_t filter;
-    uint32_t it0;   // first 1 KiB iteration of the text's 16-byte units
+    uint32_t it0;   // first wave-iteration (kIterUnits 16-byte units) of the text
     uint32_t n_it;  // iterations in the chunk
 };
 
@@ -79,6 +79,11 @@ static_assert(sizeof(BatchMatchRec) == 16, "BatchMatchRec is 16 bytes");
 // Bits of a batched candidate's offset field that hold the offset in its
 // text; the text id sits above them.
 constexpr int kBatchOffsetBits = 40;
+
+// Scan geometry: a wave-iteration loads 64 consecutive 16-byte units of text,
+// one per lane, and owns the windows that start in the first kIterUnits of
+// them; the top lanes' units only complete the windows of the lanes below.
+constexpr uint32_t kIterUnits = 62;
 
 struct BatchArgs {
     const uint8_t *texts;

@@ -3,12 +3,13 @@
 //
 // Work layout: the text is cut into 16-byte units aligned to 16 bytes in
 // memory; one lane owns the 16 windows that start in one unit, so a wave
-// covers 1 KiB of window starts per iteration and persistent waves stride
-// over the text.  A lane loads its unit plus the next (W-1)/16 units with
-// 128-bit non-temporal loads (the next iteration's units are loaded while the
-// current ones are hashed), looks every byte up once in an LDS copy of the
-// sbox, and computes its 16 hash pairs as one full window plus 15 rolling
-// slides (hash_rot_sbox_2.h:34-51).
+// covers 62 units (992 bytes) of window starts per iteration (its top two
+// lanes load the units the windows run into) and persistent waves stride over
+// the text.  A lane loads its unit with one 128-bit non-temporal load (the
+// next iteration's unit is loaded while the current one is hashed), looks its
+// bytes up in an LDS copy of the sbox, takes the sbox values of the following
+// bytes from the lanes above, and computes its 16 hash pairs as one full
+// window plus 15 rolling slides (hash_rot_sbox_2.h:34-51).
 //
 // Each window is first tested against a prefilter: P forward bits of either
 // F itself or an OR-fold of F (a fold bit is the OR of all F bits congruent
@@ -46,29 +47,26 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
     const uint8_t *base = a.text - lead;
     const uint64_t npos = a.n >= W ? a.n - W + 1 : 0;
     const uint64_t units = (lead + npos + 15u) >> 4;
-    const uint64_t iters = (units + 63u) >> 6;
+    const uint64_t iters = (units + kIterUnits - 1u) / kIterUnits;
     const uint64_t n_waves = (uint64_t)gridDim.x * (BLOCK / 64);
     uint4 *q = s_q[wv];
     uint32_t qn = 0;  // wave-uniform queue depth
 
-    constexpr int NCH = WT > 0 ? Chunks<(WT > 0 ? WT : 1)>::NCH : 1;
-    WaveChunks<NCH> cur;
+    uint4 cur = make_uint4(0u, 0u, 0u, 0u);
     uint64_t it = (uint64_t)blockIdx.x * (BLOCK / 64) + wv;
     if constexpr (WT > 0) {
-        if (it < iters) cur.load(base, lead, a.n, it << 6);
+        if (it < iters) cur = load_chunk(base, lead, a.n, it * kIterUnits + (uint64_t)lane_id());
     }
 
     for (; it < iters; it += n_waves) {
-        const uint64_t u = (it << 6) + (uint64_t)lane_id();
+        const uint64_t u = it * kIterUnits + (uint64_t)lane_id();
         uint32_t h1[kUnit], h2[kUnit];
         if constexpr (WT > 0) {
-            // issue the next iteration's loads before hashing this one
-            WaveChunks<NCH> nxt;
+            // issue the next iteration's load before hashing this one
             const uint64_t nit = it + n_waves;
-            if (nit < iters) nxt.load(base, lead, a.n, nit << 6);
-            uint4 ch[NCH];
-            cur.spread(ch);
-            hash_chunks<(WT > 0 ? WT : 1)>(ch, s_sbox, h1, h2);
+            uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
+            if (nit < iters) nxt = load_chunk(base, lead, a.n, nit * kIterUnits + (uint64_t)lane_id());
+            hash_unit<(WT > 0 ? WT : 1)>(cur, s_sbox, h1, h2);
             cur = nxt;
         } else {
             hash_unit_generic(base, lead, a.n, u, W, s_sbox, h1, h2);
@@ -98,7 +96,7 @@ hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const uint64_t W = WT ? (uint64_t)WT : a.W;
     const uint64_t npos = a.n >= W ? a.n - W + 1 : 0;
-    const uint64_t iters = (npos + 16u + 1023u) / 1024u;
+    const uint64_t iters = ((npos + 31u) / 16u + kIterUnits - 1u) / kIterUnits;
     uint64_t grid = (uint64_t)n_cu * (uint64_t)per_cu;
     const uint64_t need = (iters + (BLOCK / 64) - 1) / (BLOCK / 64);
     if (grid > need) grid = need;

@@ -3,9 +3,10 @@
 //
 // Work layout: the text is cut into 16-byte units aligned to 16 bytes in
 // memory; one lane owns the 16 windows that start in one unit.  A lane loads
-// its unit plus the next (W-1)/16 units with 128-bit non-temporal loads, looks
-// every byte up once in an LDS copy of the sbox and computes its 16 hash pairs
-// as one full window plus 15 rolling slides (hash_rot_sbox_2.h:34-51).
+// its unit with one 128-bit non-temporal load, looks its bytes up in an LDS
+// copy of the sbox, receives the sbox values of the following bytes from the
+// lanes above and computes its 16 hash pairs as one full window plus 15
+// rolling slides (hash_rot_sbox_2.h:34-51).
 #pragma once
 
 #include "ffbf_common.h"
@@ -55,59 +56,37 @@ static __device__ __forceinline__ uint4 load_chunk(const uint8_t *base, uint64_t
 // Lane L receives lane (L+1) mod 64's value (DPP wavefront rotate).
 static __device__ __forceinline__ uint32_t lane_rol1(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x134 /* wave_rol:1 */, 0xF, 0xF, false);
+    // every lane has a source in a rotation, so no "old" value is needed
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x134 /* wave_rol:1 */, 0xF, 0xF, false);
 }
-
-// A wave owns 64 consecutive units; lane L needs units L .. L+NCH-1.  Each
-// lane loads only its own unit (lane 0 also the NCH-1 units after the wave's
-// range) and the rest arrive by rotating across the wave: one HBM read per
-// text byte instead of NCH.
-template <int NCH>
-struct WaveChunks {
-    uint4 own;
-    uint4 halo[NCH > 1 ? NCH - 1 : 1];  // meaningful on lane 0 only
-
-    __device__ __forceinline__ void load(const uint8_t *base, uint64_t lead, uint64_t n, uint64_t u0_wave)
-    {
-        own = load_chunk(base, lead, n, u0_wave + (uint64_t)lane_id());
-        if (NCH > 1 && lane_id() == 0) {
-#pragma unroll
-            for (int c = 0; c < NCH - 1; c++) halo[c] = load_chunk(base, lead, n, u0_wave + 64u + (uint64_t)c);
-        }
-    }
-
-    __device__ __forceinline__ void spread(uint4 (&ch)[NCH]) const
-    {
-        ch[0] = own;
-        const bool l0 = lane_id() == 0;
-#pragma unroll
-        for (int c = 1; c < NCH; c++) {
-            const uint4 m = l0 ? halo[c - 1] : ch[c - 1];
-            ch[c] = make_uint4(lane_rol1(m.x), lane_rol1(m.y), lane_rol1(m.z), lane_rol1(m.w));
-        }
-    }
-};
 
 template <int W>
 struct Chunks {
-    static constexpr int NB = kUnit + W - 1;
+    static constexpr int NB = kUnit + W - 1;  // bytes behind a lane's 16 windows
     static constexpr int NCH = (NB + 15) / 16;
+    static_assert(NCH <= 64 - (int)kIterUnits + 1, "window longer than the lanes above can supply");
 };
 
-// Hash pairs of the 16 windows of a unit from its NCH chunks (W compile-time).
+// Hash pairs of the 16 windows of a lane's unit (W compile-time).  The lane
+// looks up the sbox values of its own 16 bytes in LDS; those of the next
+// NB-16 bytes are the lanes above's, rotated down the wave with DPP (lane L
+// gets lane L+1's values, L+2's on the second rotation): one LDS lookup per
+// text byte.  Lanes kIterUnits..63 get wrapped-around values and own no
+// windows.
 template <int W>
-static __device__ __forceinline__ void hash_chunks(const uint4 (&ch)[Chunks<W>::NCH], const uint32_t *sb, uint32_t (&h1)[kUnit],
-                                            uint32_t (&h2)[kUnit])
+static __device__ __forceinline__ void hash_unit(const uint4 own, const uint32_t *sb, uint32_t (&h1)[kUnit],
+                                                 uint32_t (&h2)[kUnit])
 {
     constexpr int NB = Chunks<W>::NB;
-    uint32_t s[NB];
+    uint32_t s[Chunks<W>::NCH * 16];
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-        const uint4 &c = ch[j >> 4];
-        const int d = (j >> 2) & 3;
-        const uint32_t w = d == 0 ? c.x : d == 1 ? c.y : d == 2 ? c.z : c.w;
+    for (int j = 0; j < 16; j++) {
+        const int d = j >> 2;
+        const uint32_t w = d == 0 ? own.x : d == 1 ? own.y : d == 2 ? own.z : own.w;
         s[j] = sb[(w >> (8 * (j & 3))) & 0xFFu];
     }
+#pragma unroll
+    for (int j = 16; j < NB; j++) s[j] = lane_rol1(s[j - 16]);
     // first window as a sum of independent rotated terms (pairs fuse into
     // three-input XORs), the other 15 by rolling
     uint32_t x1 = s[W - 1], x2 = s[W - 1];
@@ -225,7 +204,9 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
 {
     using QG = QueueGeom<BLOCK>;
     uint32_t vm = 0xFFFFu;
-    if (p0 < 0 || (uint64_t)p0 + kUnit > npos) {
+    if (lane_id() >= (int)kIterUnits) {
+        vm = 0;  // these units belong to the next wave-iteration
+    } else if (p0 < 0 || (uint64_t)p0 + kUnit > npos) {
         vm = 0;
 #pragma unroll
         for (int b = 0; b < kUnit; b++) {

@@ -189,7 +189,7 @@ typedef struct ffbf_batch_match {
 } ffbf_batch_match;
 
 typedef struct ffbf_batch_tuning {
-    uint32_t chunk_iters; /* 1 KiB iterations per work chunk (0 = 32) */
+    uint32_t chunk_iters; /* wave-iterations (992 bytes of text) per work chunk (0 = 32) */
     uint32_t probes;      /* LDS prefilter bits tested before queueing, 1..min(k,3) (0 = auto) */
     uint32_t block;       /* workgroup size, 512 or 1024 (0 = 1024) */
     uint32_t use_graph;   /* 1: replay the captured hipGraph; 0: launch stage by stage */
