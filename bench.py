@@ -423,6 +423,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (HBM counter reference) and exit")
+    ap.add_argument("--gather-sweep", action="store_true",
+                    help="only run the random-gather microbenchmark (4-byte probes into 16 KiB..2 GiB) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -442,6 +444,13 @@ def main() -> int:
         for _ in range(3):
             dst.copy_from(src, n)
         print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return 0
+
+    if args.gather_sweep:
+        for l in range(14, 32):
+            ms, rate = fa.gather_bench(l, steps=16 if l < 28 else 8)
+            print(json.dumps({"gather_array_bytes": 1 << l, "ms": round(ms, 4), "probes_per_s": round(rate, 1)}),
+                  flush=True)
         return 0
 
     if args.workload == "serve5":

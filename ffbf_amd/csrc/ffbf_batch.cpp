@@ -27,12 +27,15 @@ struct ffbf_batch {
     std::vector<ffbf_ctx *> filters;
     uint32_t W = 0, k = 0;
     uint32_t pre_words_max = 0;  // largest LDS prefilter among the filters (dynamic LDS of the scan)
+    bool has_lds = false, has_global = false;  // filters of each chunk class
 
     DevBuf<ScanArgs> d_scan;
     DevBuf<VerifyArgs> d_ver;
     DevBuf<BatchText> d_texts;
     DevBuf<BatchChunk> d_chunks;
-    DevBuf<unsigned long long> ctl;       // [0] next chunk, [1] chunks
+    // per chunk class (0: LDS prefilter, 1: prefilter in global memory), four
+    // words each: next chunk, number of chunks, index of the first, unused
+    DevBuf<unsigned long long> ctl;
     DevBuf<unsigned long long> counters;  // C_CAND, C_MATCH
     unsigned long long *h_counters = nullptr;
     DevBuf<Cand> cand;
@@ -86,15 +89,15 @@ int refresh_filters(ffbf_batch *b)
     std::vector<ScanArgs> sa(nf);
     std::vector<VerifyArgs> va(nf);
     b->pre_words_max = 0;
-    uint32_t kmin_lds = 5;
+    const bool had_lds = b->has_lds, had_global = b->has_global;
+    b->has_lds = b->has_global = false;
     for (uint32_t f = 0; f < nf; f++) {
         const ffbf_ctx *c = b->filters[f];
         const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
-        if (lds) {
-            b->pre_words_max = std::max(b->pre_words_max, c->pre_words);
-            kmin_lds = std::min(kmin_lds, c->params.k);
-        }
+        if (lds) b->pre_words_max = std::max(b->pre_words_max, c->pre_words);
+        (lds ? b->has_lds : b->has_global) = true;
     }
+    if (b->has_lds != had_lds || b->has_global != had_global) drop_graph(b);
     // probes: requested, else the smallest auto choice among LDS filters
     uint32_t p = b->tuning.probes;
     if (!p) {
@@ -147,7 +150,8 @@ int refresh_filters(ffbf_batch *b)
 // The whole batch run on b->stream (capturable: kernels and one D2H copy).
 int enqueue_run(ffbf_batch *b, bool timed)
 {
-    HIP_TRY(launch_fill32((uint32_t *)b->ctl.p, 2u, 0u, b->stream));  // next chunk = 0
+    HIP_TRY(launch_fill32((uint32_t *)b->ctl.p, 2u, 0u, b->stream));        // next chunk = 0, class 0
+    HIP_TRY(launch_fill32((uint32_t *)(b->ctl.p + 4), 2u, 0u, b->stream));  // and class 1
     HIP_TRY(launch_fill32((uint32_t *)b->counters.p, 2u * C_COUNT, 0u, b->stream));
     for (ffbf_ctx *c : b->filters) {
         HIP_TRY(launch_fill32(c->R.p, c->f_words, 0u, b->stream));
@@ -161,9 +165,15 @@ int enqueue_run(ffbf_batch *b, bool timed)
     ba.chunks = b->d_chunks.p;
     ba.filters = b->d_scan.p;
     ba.sbox = b->filters[0]->sbox.p;
-    ba.ctl = b->ctl.p;
     ba.W = b->W;
-    HIP_TRY(launch_scan_batch(ba, b->tuning.block, b->probes_eff, b->pre_words_max, b->n_cu, b->stream));
+    if (b->has_lds) {
+        ba.ctl = b->ctl.p;
+        HIP_TRY(launch_scan_batch(ba, b->tuning.block, b->probes_eff, b->pre_words_max, b->n_cu, b->stream));
+    }
+    if (b->has_global) {
+        ba.ctl = b->ctl.p + 4;
+        HIP_TRY(launch_scan_batch_global(ba, b->n_cu, b->stream));
+    }
     if (timed) HIP_TRY(hipEventRecord(b->e_scan1, b->stream));
     for (ffbf_ctx *c : b->filters) {
         PassArgs pa{};
@@ -290,7 +300,7 @@ int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch *
         HIP_TRY(hipEventCreate(&b->e_scan0));
         HIP_TRY(hipEventCreate(&b->e_scan1));
         HIP_TRY(hipHostMalloc((void **)&b->h_counters, C_COUNT * sizeof(unsigned long long), hipHostMallocDefault));
-        RET_IF(b->ctl.ensure(2, "batch control"));
+        RET_IF(b->ctl.ensure(8, "batch control"));
         RET_IF(b->counters.ensure(C_COUNT, "batch counters"));
         RET_IF(b->cand.ensure(1 << 16, "batch candidates"));
         RET_IF(b->matches.ensure(1 << 16, "batch matches"));
@@ -354,11 +364,18 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
         maxn = std::max(maxn, tx[i].n);
     }
     // chunks of chunk_iters wave-iterations (kIterUnits units of 16 bytes),
-    // ordered by filter then text
+    // ordered by chunk class, filter, then text
     std::vector<uint32_t> order(n_texts);
     for (uint32_t i = 0; i < n_texts; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t x, uint32_t y) { return tx[x].filter < tx[y].filter; });
+    // chunk class of a filter: 0 when its prefilter is held in LDS
+    auto cls = [&](uint32_t f) -> uint32_t {
+        const ffbf_ctx *c = b->filters[f];
+        return (c->mode == MODE_FOLD || c->mode == MODE_LDS) ? 0u : 1u;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        const uint32_t cx = cls(tx[x].filter), cy = cls(tx[y].filter);
+        return cx != cy ? cx < cy : tx[x].filter < tx[y].filter;
+    });
     const uint32_t ci = b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u;
     std::vector<BatchChunk> chunks;
     chunks.reserve(total / (16ull * kIterUnits * ci) + n_texts + 1);
@@ -407,7 +424,9 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     if (!chunks.empty())
         HIP_TRY(hipMemcpyAsync(b->d_chunks.p, chunks.data(), chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice,
                                b->stream));
-    const unsigned long long ctl[2] = {0ull, (unsigned long long)chunks.size()};
+    uint64_t n0 = 0;
+    while (n0 < chunks.size() && cls(chunks[n0].filter) == 0u) n0++;
+    const unsigned long long ctl[8] = {0ull, n0, 0ull, 0ull, 0ull, chunks.size() - n0, n0, 0ull};
     HIP_TRY(hipMemcpyAsync(b->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->have_texts = true;

@@ -7,7 +7,8 @@
 // chunks of wave-iterations (62 units of 16 bytes), ordered by filter, and persistent workgroups
 // pull chunks from a global counter.  A workgroup reloads its LDS prefilter
 // only when the next chunk belongs to another filter, and drains its survivor
-// queues before it does.  Candidates carry their text id in the top bits of
+// queues before it does.  Filters whose prefilter stays in global memory are
+// scanned by a second launch of small, many-wave blocks.  Candidates carry their text id in the top bits of
 // the offset.  Per text, the windows evaluated, the hash pairs, the probe
 // cascade and the feed-forward are exactly those of the single-text scan.
 #include "ffbf_scan_common.h"
@@ -16,7 +17,10 @@ namespace ffbf {
 
 namespace {
 
-template <int WT, int BLOCK, int P>
+// GLOBAL_ONLY: every filter of the launch keeps its prefilter in global
+// memory (F, or an L2-sized fold); no LDS is reserved for one, so small
+// blocks fit many waves per CU to hide the probes' round trips.
+template <int WT, int BLOCK, int P, bool GLOBAL_ONLY>
 __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
 {
     using QG = QueueGeom<BLOCK>;
@@ -30,6 +34,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
     const uint32_t W = WT ? (uint32_t)WT : b.W;
     const int wv = (int)(threadIdx.x >> 6);
     const uint32_t n_chunks = (uint32_t)b.ctl[1];
+    const uint32_t chunk0 = (uint32_t)b.ctl[2];
     uint4 *q = s_q[wv];
     uint32_t qn = 0;
     uint32_t loaded = 0xFFFFFFFFu;
@@ -42,7 +47,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         __syncthreads();
         const uint32_t ci = __builtin_amdgcn_readfirstlane(s_next);
         if (ci >= n_chunks) break;
-        const BatchChunk c = b.chunks[ci];
+        const BatchChunk c = b.chunks[chunk0 + ci];
         if (c.filter != loaded) {
             if (qn) {
                 wave_sync();
@@ -54,7 +59,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
                 qn = 0;
             }
             a = b.filters[c.filter];
-            lds = a.lds != 0u;
+            lds = !GLOBAL_ONLY && a.lds != 0u;
             __syncthreads();
             if (lds) {
                 const uint4 *src = (const uint4 *)a.pre;
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
                 hash_unit_generic(base, lead, t.n, u, W, s_sbox, h1, h2);
             }
             const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
-            if (lds)
+            if (!GLOBAL_ONLY && lds)
                 test_and_queue<true, BLOCK, P>(a, s_pre, h1, h2, p0, npos, tag, q, qn);
             else
                 test_and_queue<false, BLOCK, 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn);
@@ -131,20 +136,22 @@ __global__ __launch_bounds__(256) void k_verify_batch(BatchVerifyArgs v)
     }
 }
 
-template <int WT, int BLOCK, int P>
+template <int WT, int BLOCK, int P, bool GLOBAL_ONLY = false>
 hipError_t launch_batch_t(const BatchArgs &a, uint32_t pre_words_max, int n_cu, hipStream_t s)
 {
-    const size_t dyn = (size_t)pre_words_max * 4u;
+    const size_t dyn = GLOBAL_ONLY ? 0u : (size_t)pre_words_max * 4u;
     if (dyn > 0) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
     }
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan_batch<WT, BLOCK, P>, BLOCK, dyn);
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY>, BLOCK, dyn);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P>), dim3((uint32_t)(n_cu * per_cu)), dim3(BLOCK), dyn, s, a);
+    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY>), dim3((uint32_t)(n_cu * per_cu)), dim3(BLOCK), dyn, s,
+                       a);
     return hipGetLastError();
 }
 
@@ -174,6 +181,16 @@ hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes
     case 24: return launch_batch_w<24>(a, block, probes, pre_words_max, n_cu, s);
     case 32: return launch_batch_w<32>(a, block, probes, pre_words_max, n_cu, s);
     default: return launch_batch_w<0>(a, block, probes, pre_words_max, n_cu, s);
+    }
+}
+
+hipError_t launch_scan_batch_global(const BatchArgs &a, int n_cu, hipStream_t s)
+{
+    switch (a.W) {
+    case 19: return launch_batch_t<19, 256, 1, true>(a, 0u, n_cu, s);
+    case 24: return launch_batch_t<24, 256, 1, true>(a, 0u, n_cu, s);
+    case 32: return launch_batch_t<32, 256, 1, true>(a, 0u, n_cu, s);
+    default: return launch_batch_t<0, 256, 1, true>(a, 0u, n_cu, s);
     }
 }
 

@@ -91,7 +91,7 @@ struct BatchArgs {
     const BatchChunk *chunks;
     const ScanArgs *filters;  // per filter; cand/counters are the batch's
     const uint32_t *sbox;
-    unsigned long long *ctl;  // [0] next chunk, [1] number of chunks
+    unsigned long long *ctl;  // [0] next chunk, [1] number of chunks, [2] index of the first
     uint32_t W;
 };
 
@@ -153,6 +153,8 @@ hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
 
 hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
                              hipStream_t s);
+// the same over filters whose prefilter is in global memory (256-thread blocks, no LDS prefilter)
+hipError_t launch_scan_batch_global(const BatchArgs &a, int n_cu, hipStream_t s);
 hipError_t launch_verify_batch(const BatchVerifyArgs &a, int n_cu, hipStream_t s);
 hipError_t sort_batch_matches(BatchMatchRec *m, uint64_t n, uint64_t *keys_a, uint64_t *keys_b, void *temp,
                               size_t temp_bytes, int id_shift, int text_shift, int end_bit, hipStream_t s);

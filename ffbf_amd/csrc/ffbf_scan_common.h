@@ -28,6 +28,13 @@ struct QueueGeom<1024> {
     static constexpr int cap = 120;
     static constexpr int drain_at = 57;
 };
+// 512-thread blocks: what LDS is left beside a 128 KiB prefilter, for filters
+// that pass many windows
+template <>
+struct QueueGeom<512> {
+    static constexpr int cap = 240;
+    static constexpr int drain_at = 64;
+};
 // the global-prefilter scan passes ~1 window in 8: room for two drains' worth
 template <>
 struct QueueGeom<256> {
@@ -176,15 +183,25 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
             for (int j = 2; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
         }
     }
-    if (!a.pre_exact && ok) {
-        uint32_t words[5], idx[5];
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-            idx[j] = fwd_idx(j, h1, h2) & a.mask;
-            words[j] = ((uint32_t)j < a.k) ? a.F[idx[j] >> 5] : 0xFFFFFFFFu;
+    if (!a.pre_exact) {
+        // F itself: its first bit for every window still passing, the rest
+        // only where that bit is set.  F is too big for L2 whenever a fold is
+        // in use, and its fill is low, so this takes ~1 random access per
+        // window instead of k
+        if (ok) {
+            const uint32_t i0 = h1 & a.mask;
+            ok &= (a.F[i0 >> 5] >> (i0 & 31u)) & 1u;
         }
+        if (ok && a.k > 1u) {
+            uint32_t words[5], idx[5];
 #pragma unroll
-        for (int j = 0; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
+            for (int j = 1; j < 5; j++) {
+                idx[j] = fwd_idx(j, h1, h2) & a.mask;
+                words[j] = ((uint32_t)j < a.k) ? a.F[idx[j] >> 5] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int j = 1; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
+        }
     }
     const bool hit = ok != 0u;
     const uint64_t off = ((uint64_t)e.y << 32) | e.x;
