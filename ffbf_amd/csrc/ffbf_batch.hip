@@ -37,6 +37,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
     const uint32_t chunk0 = (uint32_t)b.ctl[2];
     uint4 *q = s_q[wv];
     uint32_t qn = 0;
+    PendingCand pend;
     uint32_t loaded = 0xFFFFFFFFu;
     ScanArgs a{};
     bool lds = false;
@@ -52,12 +53,13 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
             if (qn) {
                 wave_sync();
                 if (lds)
-                    drain<true, P>(q, qn, a, s_pre);
+                    drain<true, P>(q, qn, a, s_pre, pend);
                 else
-                    drain<false, 1>(q, qn, a, s_pre);
+                    drain<false, 1>(q, qn, a, s_pre, pend);
                 wave_sync();
                 qn = 0;
             }
+            if (loaded != 0xFFFFFFFFu) flush_pending(pend, a);
             a = b.filters[c.filter];
             lds = !GLOBAL_ONLY && a.lds != 0u;
             __syncthreads();
@@ -96,18 +98,19 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
             }
             const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
             if (!GLOBAL_ONLY && lds)
-                test_and_queue<true, BLOCK, P>(a, s_pre, h1, h2, p0, npos, tag, q, qn);
+                test_and_queue<true, BLOCK, P>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
             else
-                test_and_queue<false, BLOCK, 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn);
+                test_and_queue<false, BLOCK, 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
         }
     }
     if (qn) {
         wave_sync();
         if (lds)
-            drain<true, P>(q, qn, a, s_pre);
+            drain<true, P>(q, qn, a, s_pre, pend);
         else
-            drain<false, 1>(q, qn, a, s_pre);
+            drain<false, 1>(q, qn, a, s_pre, pend);
     }
+    if (loaded != 0xFFFFFFFFu) flush_pending(pend, a);
 }
 
 // Exact verification of a batch: the candidate's text id selects the text

@@ -100,9 +100,9 @@ uint32_t l2_fold_log2(ffbf_ctx *c)
 
 // Picks how the scan pre-tests windows, after F changed or a mode was
 // requested.  Automatic choice: the whole filter in LDS when it fits; else
-// the LDS fold when its expected fill leaves few windows for F; else F
-// itself when it fits one XCD's L2, else an L2-sized fold.  The folds are
-// computed on the stream; nothing is read back.
+// the cheapest of the LDS fold, F probed directly, and (F beyond one XCD's
+// L2) an L2-sized fold, by an expected-cost model over the analytic fills.
+// The folds are computed on the stream; nothing is read back.
 int choose_mode(ffbf_ctx *c)
 {
     const uint32_t max_words = ffbf::max_fold_words(c->device);
@@ -134,28 +134,48 @@ int choose_mode(ffbf_ctx *c)
             words = 1u << (c->fold_log2_req - 5);
         }
         words = (uint32_t)std::min<uint64_t>(words, c->f_words);
-        RET_IF(c->fold.ensure(words, "fold"));
-        HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
-        c->pre_words = words;
-        c->pre_fill = fold_fill(c, (uint64_t)words * 32u);
-        // the LDS fold pays while few windows get past its first two bits;
-        // otherwise probe F directly if it fits in one XCD's L2, else a fold
-        // of F sized to L2
-        const double pass2 = c->pre_fill * (k > 1 ? c->pre_fill : 1.0);
-        if (c->mode_req == ffbf::MODE_FOLD || pass2 < 0.3) {
-            mode = ffbf::MODE_FOLD;
-        } else {
-            const uint32_t gl2 = l2_fold_log2(c);
-            if (c->f_words * 4ull <= (4ull << 20)) {
-                mode = ffbf::MODE_GLOBAL;
-                c->pre_words = (uint32_t)c->f_words;
-            } else {
-                const uint32_t words = 1u << (gl2 - 5);
-                RET_IF(c->gfold.ensure(words, "global fold"));
-                HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, words, c->stream));
-                mode = ffbf::MODE_GFOLD;
-                c->pre_words = words;
+        const double ff = fold_fill(c, (uint64_t)words * 32u);
+        uint32_t choice = ffbf::MODE_FOLD;
+        uint32_t gl2 = 0;
+        if (c->mode_req != ffbf::MODE_FOLD) {
+            // Expected scan cost per window (picoseconds of the whole chip),
+            // fitted to MI355X measurements: the LDS stage (hash + up to 3
+            // fold bits) ~1.25; one random probe ~3.8 while its array fits
+            // one XCD's L2 (265 G probes/s), ~18 beyond (55 G/s); ~0.8 per
+            // window queued past the up-front bits.
+            constexpr double kLds = 1.25, kL2 = 3.8, kFar = 18.0, kQueue = 0.8;
+            const bool f_in_l2 = c->f_words * 4ull <= (4ull << 20);
+            const double far = f_in_l2 ? kL2 : kFar;
+            const double f = f_fill(c);
+            const double cost_fold =
+                kLds + std::pow(ff, (double)k) * far + std::pow(ff, (double)std::min(3u, k)) * kQueue;
+            const double cost_global = (1.0 + f + f * f) * far;
+            double cost_gfold = 1e30;
+            if (!f_in_l2) {
+                gl2 = l2_fold_log2(c);
+                const double g = fold_fill(c, 1ull << gl2);
+                cost_gfold = (1.0 + g + g * g) * kL2 + std::pow(g, (double)k) * kFar;
             }
+            if (cost_global < cost_fold && cost_global <= cost_gfold)
+                choice = ffbf::MODE_GLOBAL;
+            else if (cost_gfold < cost_fold)
+                choice = ffbf::MODE_GFOLD;
+        }
+        if (choice == ffbf::MODE_FOLD) {
+            RET_IF(c->fold.ensure(words, "fold"));
+            HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
+            mode = ffbf::MODE_FOLD;
+            c->pre_words = words;
+            c->pre_fill = ff;
+        } else if (choice == ffbf::MODE_GLOBAL) {
+            mode = ffbf::MODE_GLOBAL;
+            c->pre_words = (uint32_t)c->f_words;
+        } else {
+            const uint32_t gw = 1u << (gl2 - 5);
+            RET_IF(c->gfold.ensure(gw, "global fold"));
+            HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, gw, c->stream));
+            mode = ffbf::MODE_GFOLD;
+            c->pre_words = gw;
         }
     }
     RET_IF(st);

@@ -51,6 +51,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
     const uint64_t n_waves = (uint64_t)gridDim.x * (BLOCK / 64);
     uint4 *q = s_q[wv];
     uint32_t qn = 0;  // wave-uniform queue depth
+    PendingCand pend;
 
     uint4 cur = make_uint4(0u, 0u, 0u, 0u);
     uint64_t it = (uint64_t)blockIdx.x * (BLOCK / 64) + wv;
@@ -73,12 +74,13 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
         }
 
         // window offset of slot b is 16u + b - lead
-        test_and_queue<LDSPRE, BLOCK, P>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos, 0, q, qn);
+        test_and_queue<LDSPRE, BLOCK, P>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos, 0, q, qn, pend);
     }
     if (qn) {
         wave_sync();
-        drain<LDSPRE, P>(q, qn, a, s_pre);
+        drain<LDSPRE, P>(q, qn, a, s_pre, pend);
     }
+    flush_pending(pend, a);
 }
 
 template <int WT, bool LDSPRE, int BLOCK, int P>

@@ -146,11 +146,42 @@ static __device__ __forceinline__ void hash_unit_generic(const uint8_t *base, ui
     }
 }
 
+// A lane's candidate not yet appended to the global list.  The list's counter
+// is a device-wide atomic that every appending wave hits, so appends go out
+// in bulk: a wave flushes all its held candidates (one atomic) only when a
+// new hit lands on a lane that still holds one, and at the end of its work.
+struct PendingCand {
+    uint64_t off = 0;
+    uint32_t h1 = 0, h2 = 0;
+    bool valid = false;
+};
+
+// Called by the whole wave.
+static __device__ __forceinline__ void flush_pending(PendingCand &pc, const ScanArgs &a)
+{
+    append_cand(pc.valid, pc.off, pc.h1, pc.h2, a.cand, a.cand_cap, a.counters + C_CAND);
+    pc.valid = false;
+}
+
+// Called by the whole wave.
+static __device__ __forceinline__ void stage_cand(PendingCand &pc, bool hit, uint64_t off, uint32_t h1, uint32_t h2,
+                                                  const ScanArgs &a)
+{
+    if (__ballot(hit && pc.valid)) flush_pending(pc, a);
+    if (hit) {
+        pc.off = off;
+        pc.h1 = h1;
+        pc.h2 = h2;
+        pc.valid = true;
+    }
+}
+
 // Finishes up to 64 queued windows, one per lane: the prefilter bits not yet
 // tested, then (for a fold) the k forward bits of F, then feed-forward and
-// candidate append.  Called by the whole wave.
+// candidate staging.  Called by the whole wave.
 template <bool LDSPRE, int P>
-static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const ScanArgs &a, const uint32_t *s_pre)
+static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const ScanArgs &a, const uint32_t *s_pre,
+                                             PendingCand &pend)
 {
     const bool act = (uint32_t)lane_id() < cnt;
     uint4 e = make_uint4(0u, 0u, 0u, 0u);
@@ -206,7 +237,7 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
     const bool hit = ok != 0u;
     const uint64_t off = ((uint64_t)e.y << 32) | e.x;
     feed_forward(hit, h1, h2, a.k, a.mask, a.R);
-    append_cand(hit, off, h1, h2, a.cand, a.cand_cap, a.counters + C_CAND);
+    stage_cand(pend, hit, off, h1, h2, a);
 }
 
 // Prefilter test of one lane's 16 windows (window b starts at text offset
@@ -217,7 +248,7 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
 template <bool LDSPRE, int BLOCK, int P>
 static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const uint32_t *s_pre, const uint32_t (&h1)[kUnit],
                                                       const uint32_t (&h2)[kUnit], int64_t p0, uint64_t npos,
-                                                      uint64_t tag, uint4 *q, uint32_t &qn)
+                                                      uint64_t tag, uint4 *q, uint32_t &qn, PendingCand &pend)
 {
     using QG = QueueGeom<BLOCK>;
     uint32_t vm = 0xFFFFu;
@@ -277,7 +308,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         while (qn >= (uint32_t)QG::drain_at) {
             const uint32_t cnt = qn < 64u ? qn : 64u;
             wave_sync();
-            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre);
+            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre, pend);
             wave_sync();
             qn -= cnt;
         }
@@ -298,7 +329,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         if (qn >= (uint32_t)QG::drain_at) {
             const uint32_t cnt = qn < 64u ? qn : 64u;
             wave_sync();
-            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre);
+            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre, pend);
             wave_sync();
             qn -= cnt;
         }

@@ -23,13 +23,15 @@ W, K = 24, 3
 
 
 def make_filters(sbox, specs, seed):
-    """specs: list of (n_patterns, log2_bits, alphabet); returns matchers and pattern sets."""
+    """specs: list of (n_patterns, log2_bits, alphabet, scan mode); returns matchers and pattern sets."""
     out = []
-    for i, (n, log2, alpha) in enumerate(specs):
+    for i, (n, log2, alpha, mode) in enumerate(specs):
         b, off = fa.gen_patterns(alpha, seed + i, n, W, 60)
         m = fa.Matcher(W, K, log2, sbox)
         m.set_patterns(b, off)
         m.build()
+        m.set_scan_mode(mode)
+        assert m.timings()["scan_mode"] == mode
         out.append((m, b, off, log2))
     return out
 
@@ -70,10 +72,10 @@ def as_rows(m):
 
 
 SPECS = [
-    (500, 14, fa.ALPHA_DNA_GC40),      # whole filter in LDS
-    (3000, 22, fa.ALPHA_ASCII),        # LDS fold
-    (20000, 18, fa.ALPHA_DNA_GC40),    # saturated-ish: F probed in L2
-    (60000, 26, fa.ALPHA_ASCII),       # L2-sized global fold
+    (500, 14, fa.ALPHA_DNA_GC40, fa.MODE_LDS),       # whole filter in LDS
+    (3000, 22, fa.ALPHA_ASCII, fa.MODE_FOLD),        # LDS fold
+    (20000, 18, fa.ALPHA_DNA_GC40, fa.MODE_GLOBAL),  # F probed in global memory
+    (60000, 26, fa.ALPHA_ASCII, fa.MODE_GFOLD),      # L2-sized global fold
 ]
 
 
@@ -163,7 +165,7 @@ def test_batch_candidate_overflow():
     say("batch overflow")
     sbox = fa.sbox_from_seed(80)
     # a saturated 1 Kibit filter: almost every window is a candidate
-    filters = make_filters(sbox, [(2000, 10, fa.ALPHA_DNA_GC40), (100, 14, fa.ALPHA_ASCII)], 600)
+    filters = make_filters(sbox, [(2000, 10, fa.ALPHA_DNA_GC40, fa.MODE_LDS), (100, 14, fa.ALPHA_ASCII, fa.MODE_LDS)], 600)
     texts, off, route = make_texts(filters, 12, 9, max_len=150_000)
     dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
     expect, _ = oracle_batch(sbox, filters, texts, route)

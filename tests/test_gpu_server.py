@@ -23,14 +23,17 @@ def say(*a):
 def make_filters(sbox):
     # whole filter in LDS, LDS fold, F in L2, L2-sized global fold: each mode
     # is copied into the server's replicas differently
-    specs = [(800, 14, fa.ALPHA_DNA_GC40), (5000, 22, fa.ALPHA_ASCII), (40000, 20, fa.ALPHA_DNA_GC40),
-             (200000, 22, fa.ALPHA_DNA_GC40), (1_000_000, 26, fa.ALPHA_ASCII)]
+    specs = [(800, 14, fa.ALPHA_DNA_GC40, fa.MODE_LDS), (5000, 22, fa.ALPHA_ASCII, fa.MODE_FOLD),
+             (40000, 20, fa.ALPHA_DNA_GC40, fa.MODE_GLOBAL), (200000, 22, fa.ALPHA_DNA_GC40, fa.MODE_AUTO),
+             (1_000_000, 26, fa.ALPHA_ASCII, fa.MODE_GFOLD)]
     out = []
-    for i, (n, log2, alpha) in enumerate(specs):
+    for i, (n, log2, alpha, mode) in enumerate(specs):
         b, off = fa.gen_patterns(alpha, 900 + i, n, W, 70)
         m = fa.Matcher(W, K, log2, sbox)
         m.set_patterns(b, off)
         m.build()
+        m.set_scan_mode(mode)
+        assert mode == fa.MODE_AUTO or m.timings()["scan_mode"] == mode
         out.append((m, b, off, log2, alpha))
     return out
 
