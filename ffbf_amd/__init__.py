@@ -74,6 +74,7 @@ class Timings(ctypes.Structure):
         ("fold_log2_bits", ctypes.c_uint32),
         ("probes", ctypes.c_uint32),
         ("block", ctypes.c_uint32),
+        ("ingest_ms", ctypes.c_float),
     ]
 
     def as_dict(self) -> dict:
@@ -111,6 +112,12 @@ ABI = {
     "ffbf_build": (_i32, [_P]),
     "ffbf_scan": (_i32, [_P, _P, _u64]),
     "ffbf_scan_device": (_i32, [_P, _P, _u64]),
+    "ffbf_scan_fd": (_i32, [_P, _i32]),
+    "ffbf_scan_file": (_i32, [_P, _c.c_char_p]),
+    "ffbf_run_file": (_i32, [_P, _c.c_char_p, _pu64]),
+    "ffbf_set_ingest": (_i32, [_P, _u64, _u32, _u32]),
+    "ffbf_write_survivors": (_i32, [_P, _c.c_char_p, _c.c_char_p]),
+    "ffbf_write_candidate_lines": (_i32, [_P, _P, _u64, _c.c_char_p]),
     "ffbf_run_device": (_i32, [_P, _P, _u64, _pu64]),
     "ffbf_pattern_pass": (_i32, [_P, _pu64]),
     "ffbf_verify": (_i32, [_P, _pu64]),
@@ -402,6 +409,38 @@ class Matcher:
     def scan_device(self, ptr: int, n: int) -> int:
         _check(_lib.ffbf_scan_device(self._h, ptr, n))
         return self.n_candidates()
+
+    def scan_fd(self, fd: int) -> int:
+        """Scan everything readable from fd (a file from its position, a pipe)."""
+        _check(_lib.ffbf_scan_fd(self._h, fd))
+        return self.n_candidates()
+
+    def scan_file(self, path) -> int:
+        _check(_lib.ffbf_scan_file(self._h, os.fsencode(path)))
+        return self.n_candidates()
+
+    def run_file(self, path) -> int:
+        nm = _c.c_uint64(0)
+        _check(_lib.ffbf_run_file(self._h, os.fsencode(path), _c.byref(nm)))
+        return nm.value
+
+    def set_ingest(self, chunk_bytes: int = 0, depth: int = 0, threads: int = 0) -> None:
+        _check(_lib.ffbf_set_ingest(self._h, chunk_bytes, depth, threads))
+
+    def write_survivors(self, patterns_path, index_path=None) -> None:
+        """Surviving patterns (one per line) and their 0-based ids, as the
+        reference's filtered-phrases and __index_ files."""
+        _check(_lib.ffbf_write_survivors(self._h, os.fsencode(patterns_path),
+                                         None if index_path is None else os.fsencode(index_path)))
+
+    def write_candidate_lines(self, out_path, text=None) -> None:
+        """Lines of the scanned text holding a candidate window (the filtered
+        corpus); text: host copy of the scanned text, else read back."""
+        if text is None:
+            _check(_lib.ffbf_write_candidate_lines(self._h, None, 0, os.fsencode(out_path)))
+        else:
+            t = np.frombuffer(text, dtype=np.uint8) if isinstance(text, (bytes, bytearray)) else np.ascontiguousarray(text)
+            _check(_lib.ffbf_write_candidate_lines(self._h, _ptr(t), t.size, os.fsencode(out_path)))
 
     def run_device(self, ptr: int, n: int) -> int:
         nm = _c.c_uint64(0)

@@ -200,15 +200,25 @@ uint32_t scan_block(ffbf_ctx *c)
     return 256u;
 }
 
-int enqueue_scan(ffbf_ctx *c)
+// Clears R and the candidate count before the first piece of a scan.
+int enqueue_scan_begin(ffbf_ctx *c)
 {
     RET_IF(record(c, EV_S0));
     HIP_TRY(ffbf::launch_fill32(c->R.p, c->f_words, 0u, c->stream));
     RET_IF(clear_counters(c, ffbf::C_CAND, 1));
     RET_IF(record(c, EV_S1));
+    return FFBF_OK;
+}
+
+// Scans the windows of text[0..n) (device memory), recording their offsets
+// plus base: a text can be scanned piece by piece, each piece starting at the
+// first window the previous one did not cover and carrying its W-1 byte tail.
+int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base)
+{
     ffbf::ScanArgs a{};
-    a.text = c->text;
-    a.n = c->text_n;
+    a.text = text;
+    a.n = n;
+    a.off_base = base;
     a.F = c->F.p;
     a.R = c->R.p;
     a.sbox = c->sbox.p;
@@ -229,6 +239,13 @@ int enqueue_scan(ffbf_ctx *c)
     a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
     HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
+    return FFBF_OK;
+}
+
+int enqueue_scan(ffbf_ctx *c)
+{
+    RET_IF(enqueue_scan_begin(c));
+    RET_IF(enqueue_scan_range(c, c->text, c->text_n, 0));
     RET_IF(record(c, EV_S2));
     return FFBF_OK;
 }
@@ -445,6 +462,7 @@ void ffbf_destroy(ffbf_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ingest_release(c);
     c->F.release();
     c->R.release();
     c->sbox.release();
@@ -578,10 +596,7 @@ int ffbf_scan(ffbf_ctx *c, const uint8_t *text, uint64_t n)
 {
     RET_IF(check_ctx(c));
     if (!text && n) return fail(FFBF_EINVAL, "null text");
-    RET_IF(c->owned_text.ensure(n, "text"));
-    if (n) HIP_TRY(hipMemcpyAsync(c->owned_text.p, text, n, hipMemcpyHostToDevice, c->stream));
-    RET_IF(set_text(c, c->owned_text.p, n));
-    return do_scan(c);
+    return scan_host(c, text, n);
 }
 
 int ffbf_pattern_pass(ffbf_ctx *c, uint64_t *n_survivors)
@@ -657,10 +672,10 @@ int ffbf_match_text(ffbf_ctx *c, const uint8_t *text, uint64_t n, ffbf_match **o
     *out = nullptr;
     *n_out = 0;
     if (!text && n) return fail(FFBF_EINVAL, "null text");
-    RET_IF(c->owned_text.ensure(n, "text"));
-    if (n) HIP_TRY(hipMemcpyAsync(c->owned_text.p, text, n, hipMemcpyHostToDevice, c->stream));
     uint64_t nm = 0;
-    RET_IF(ffbf_run_device(c, c->owned_text.p, n, &nm));
+    RET_IF(scan_host(c, text, n));
+    RET_IF(ffbf_pattern_pass(c, nullptr));
+    RET_IF(ffbf_verify(c, &nm));
     RET_IF(load_sorted_matches(c));
     ffbf_match *m = (ffbf_match *)std::malloc(std::max<uint64_t>(nm, 1) * sizeof(ffbf_match));
     if (!m) return fail(FFBF_ENOMEM, "match list allocation");
@@ -778,6 +793,7 @@ int ffbf_get_timings(ffbf_ctx *c, ffbf_timings *out)
     out->fold_log2_bits = c->mode == ffbf::MODE_GLOBAL ? 0u : pow2_at_least((uint64_t)c->pre_words * 32u);
     out->probes = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
     out->block = scan_block(c);
+    out->ingest_ms = (float)c->ingest_ms;
     return FFBF_OK;
 }
 

@@ -110,9 +110,34 @@ struct ffbf_ctx {
     bool patterns = false, built = false, scanned = false, passed = false, verified = false;
     hipEvent_t ev[ffbf::EV_N] = {};
     bool ev_recorded[ffbf::EV_N] = {};
+
+    // text ingest from host memory and files (ffbf_ingest.cpp)
+    hipStream_t copy_stream = nullptr;
+    uint8_t *staging[4] = {};         // pinned pieces of the text on their way to the device
+    hipEvent_t ev_staged[4] = {};     // H2D copy of each staging buffer done
+    bool staged_pending[4] = {};
+    uint64_t staging_bytes = 0;       // size of each allocated staging buffer
+    uint32_t staging_depth = 0;       // staging buffers allocated
+    uint64_t ingest_chunk = 64ull << 20;
+    uint32_t ingest_depth = 4;
+    uint32_t ingest_threads = 8;      // host threads copying or reading a piece
+    double ingest_ms = 0.0;           // last ingest, host clock, first byte to last candidate
 };
 
 namespace ffbf {
+// Context internals shared by the implementation files (ffbf_capi.cpp).
+int check_ctx(ffbf_ctx *c);
+int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n);
+int record(ffbf_ctx *c, int e);
+int sync_counters(ffbf_ctx *c);
+int enqueue_scan_begin(ffbf_ctx *c);
+int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base);
+int do_scan(ffbf_ctx *c);
+// Text ingest (ffbf_ingest.cpp): host memory through the staging pipeline,
+// and release of the pipeline's buffers and stream.
+int scan_host(ffbf_ctx *c, const uint8_t *text, uint64_t n);
+void ingest_release_staging(ffbf_ctx *c);
+void ingest_release(ffbf_ctx *c);
 // The stream a batch object queues its work on (ffbf_batch.cpp).
 hipStream_t batch_stream(ffbf_batch *b);
 // Stage launchers shared with the batch path (ffbf_capi.cpp).

@@ -51,6 +51,7 @@ struct ScanArgs {
     uint32_t P;          // forward bits tested against the prefilter before queueing (1..k; 1 for global)
     uint32_t pre_exact;  // prefilter is F itself: no further F probes
     uint32_t lds;        // prefilter is copied to LDS (batched scan: per filter)
+    uint64_t off_base;   // added to every recorded offset (a text scanned in pieces)
 };
 
 // Batched scan: many texts (concatenated in one device buffer), each routed

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
         }
 
         // window offset of slot b is 16u + b - lead
-        test_and_queue<LDSPRE, BLOCK, P>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos, 0, q, qn, pend);
+        test_and_queue<LDSPRE, BLOCK, P>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos, a.off_base, q, qn, pend);
     }
     if (qn) {
         wave_sync();

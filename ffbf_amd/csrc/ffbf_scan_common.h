@@ -243,8 +243,8 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
 // Prefilter test of one lane's 16 windows (window b starts at text offset
 // p0+b; only offsets in [0, npos) are real) and wave-aggregated push of the
 // survivors onto the wave's LDS queue, draining it 64 at a time.  `tag` is
-// OR-ed into the recorded offsets (the batched scan stores the text id in the
-// top bits).  Called by the whole wave.
+// added to the recorded offsets (the offset of a text piece, or the batched
+// scan's text id in the top bits).  Called by the whole wave.
 template <bool LDSPRE, int BLOCK, int P>
 static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const uint32_t *s_pre, const uint32_t (&h1)[kUnit],
                                                       const uint32_t (&h2)[kUnit], int64_t p0, uint64_t npos,
@@ -293,8 +293,8 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         // room for every survivor: each lane writes its own at its rank in
         // the wave (exclusive prefix of the per-lane counts), no ballots
         uint4 *dst = q + (qn + incl - mine);
-        // tag + (p0 + b) == tag | (p0 + b) for the real windows (0 <= p0+b <
-        // 2^40); p0 itself may be negative on a text's first unit
+        // p0 itself may be negative on a text's first unit; p0 + b is not
+        // for the windows written
         const uint64_t pt = tag + (uint64_t)p0;
 #pragma unroll
         for (int b = 0; b < kUnit; b++) {
@@ -322,7 +322,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         const bool pass = (pm >> b) & 1u;
         const uint64_t bal = __ballot(pass);
         if (pass) {
-            const uint64_t p = (uint64_t)(p0 + (int64_t)b) | tag;
+            const uint64_t p = tag + (uint64_t)(p0 + (int64_t)b);
             q[qn + lanes_below(bal)] = make_uint4((uint32_t)p, (uint32_t)(p >> 32), h1[b], h2[b]);
         }
         qn += (uint32_t)__builtin_popcountll(bal);

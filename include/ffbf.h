@@ -11,15 +11,17 @@
  *
  * Reference interfaces replaced (efficient/ffbf):
  *   setup_bloom_search        src/gpgpu/greptest.cu:455-474   -> ffbf_build
- *   executeGrepOverlap        src/gpgpu/greptest.cu:505-576   -> ffbf_scan / ffbf_scan_device
+ *   executeGrepOverlap        src/gpgpu/greptest.cu:505-576   -> ffbf_scan_file / ffbf_scan / ffbf_scan_device
  *   executePatternFiltering   src/gpgpu/greptest.cu:578-596   -> ffbf_pattern_pass
  *   bvDump                    src/gpgpu/greptest.cu:341-351   -> ffbf_get_filter
  *   positions bitmap copy-out src/gpgpu/greptest.cu:697-704   -> ffbf_get_candidates
  *   patterns bitmap copy-out  src/gpgpu/greptest.cu:705-707   -> ffbf_get_survivors
  *   grep -Ff hand-off         src/rabin-karp/README:49-52     -> ffbf_verify / ffbf_get_matches
  *   Filter::operator<<        src/rabin-karp/ffbf.cpp:268-283 -> ffbf_set_patterns + ffbf_build
- *   Filter::processFile       src/rabin-karp/ffbf.cpp:310-392 -> ffbf_scan
- *   Filter::filterPhrases     src/rabin-karp/ffbf.cpp:394-485 -> ffbf_pattern_pass
+ *   Filter::processFile       src/rabin-karp/ffbf.cpp:310-392 -> ffbf_scan / ffbf_scan_fd
+ *   Filter::filterPhrases     src/rabin-karp/ffbf.cpp:394-485 -> ffbf_pattern_pass + ffbf_write_survivors
+ *   printpositions            src/gpgpu/greptest.cu:353-398   -> ffbf_write_candidate_lines
+ *   printpatterns             src/gpgpu/greptest.cu:400-426   -> ffbf_write_survivors
  *   Filter::save/loadFilterToFile  src/rabin-karp/ffbf.cpp:487-527 -> ffbf_filter_save / ffbf_filter_load
  *
  * Conventions: every call returns FFBF_OK (0) or a negative status and never
@@ -79,6 +81,8 @@ typedef struct ffbf_timings {
     uint32_t fold_log2_bits; /* prefilter size (0 in mode 0) */
     uint32_t probes;         /* forward bits tested against the prefilter before queueing */
     uint32_t block;          /* scan workgroup size */
+    float ingest_ms;         /* last scan of host memory or a file: first byte read to last
+                                candidate recorded (host clock, includes the overlapped copies) */
 } ffbf_timings;
 
 typedef struct ffbf_ctx ffbf_ctx;
@@ -103,9 +107,32 @@ int ffbf_set_patterns_lines(ffbf_ctx *ctx, const char *buf, uint64_t len);
 /* Builds F from the loaded patterns (greptest.cu:455-474, :96-123). */
 int ffbf_build(ffbf_ctx *ctx);
 
-/* Scans a host text: copies it to device memory owned by the context, clears
- * R, scans every offset in [0, n-W] (executeGrepOverlap, greptest.cu:505-576). */
+/* Scans a host text: clears R and scans every offset in [0, n-W]
+ * (executeGrepOverlap, greptest.cu:505-576).  The text moves into device
+ * memory owned by the context through pinned staging buffers, piece by
+ * piece; each piece's copy overlaps the scan of the windows already complete
+ * (a window straddling two pieces is scanned once both have landed). */
 int ffbf_scan(ffbf_ctx *ctx, const uint8_t *text, uint64_t n);
+
+/* The same for everything readable from a file descriptor, from its current
+ * position to the end (Filter::processFile(int fd), ffbf.cpp:310-392, which
+ * the reference's CLI runs on stdin).  Regular files are read in parallel
+ * slices; pipes and sockets sequentially, the device text growing as needed. */
+int ffbf_scan_fd(ffbf_ctx *ctx, int fd);
+
+/* The same for a file by name (executeGrepOverlap(filename, ...),
+ * greptest.cu:505-576). */
+int ffbf_scan_file(ffbf_ctx *ctx, const char *path);
+
+/* ffbf_scan_file, then the pattern pass and verification; n_matches may be
+ * NULL (matches are then read with ffbf_get_matches). */
+int ffbf_run_file(ffbf_ctx *ctx, const char *path, uint64_t *n_matches);
+
+/* Ingest tuning for ffbf_scan, ffbf_scan_fd and ffbf_scan_file: bytes per
+ * staging buffer (1 MiB..4 GiB; default 64 MiB), staging buffers (2..4;
+ * default 4) and host threads filling each buffer (1..64; default 8).  0
+ * keeps a value. */
+int ffbf_set_ingest(ffbf_ctx *ctx, uint64_t chunk_bytes, uint32_t depth, uint32_t threads);
 
 /* Scans a text already in device memory of the context's device. */
 int ffbf_scan_device(ffbf_ctx *ctx, const uint8_t *dev_text, uint64_t n);
@@ -131,6 +158,21 @@ int ffbf_get_matches(ffbf_ctx *ctx, ffbf_match *out, uint64_t cap, uint64_t *n);
  * *out is allocated by the library (free with ffbf_free_matches). */
 int ffbf_match_text(ffbf_ctx *ctx, const uint8_t *text, uint64_t n, ffbf_match **out, uint64_t *n_out);
 void ffbf_free_matches(ffbf_match *matches);
+
+/* The surviving patterns of the last pattern pass, one per line in id order,
+ * written to patterns_path, and their 0-based ids, one per line, to
+ * index_path (NULL: not written) -- the filtered phrases and "__index_" files
+ * of Filter::filterPhrases (ffbf.cpp:394-485, named at ffbf.cpp:537-541;
+ * printpatterns, greptest.cu:400-426, writes the first only). */
+int ffbf_write_survivors(ffbf_ctx *ctx, const char *patterns_path, const char *index_path);
+
+/* Every line of the last scanned text that holds a candidate window start,
+ * once each, in text order, each followed by '\n' (printpositions,
+ * greptest.cu:353-398): the filtered corpus the reference hands to grep -Ff.
+ * A candidate that starts on a '\n' byte prints nothing; a last line without
+ * '\n' ends at the end of the text.  text/n: the caller's host copy of the
+ * scanned text, or NULL/0 to read it back from the device. */
+int ffbf_write_candidate_lines(ffbf_ctx *ctx, const uint8_t *text, uint64_t n, const char *out_path);
 
 /* Filter dump (bvDump, greptest.cu:341-351): n_words must be >= 2^log2_bits/32. */
 int ffbf_get_filter(ffbf_ctx *ctx, int which, uint32_t *out, uint64_t n_words);

@@ -18,6 +18,9 @@ reference, which does not build here):
   verify            independent naive matcher (brute_matches) and planted
                     occurrences
   save/load deltas  round trip and hand-built vectors
+  candidate_lines / survivor_files   hand-built texts and pattern sets
+                    (newline runs, a line without '\n', a candidate on a line
+                    break, several candidates in one line)
 """
 from __future__ import annotations
 
@@ -185,3 +188,40 @@ def filter_load_deltas(deltas: np.ndarray, log2_bits: int) -> np.ndarray:
     d = np.ascontiguousarray(deltas, dtype=np.uint32)
     _lib.orc_filter_load_deltas(_ptr(d), d.size, log2_bits, _ptr(F))
     return F
+
+
+def candidate_lines(text, cand) -> bytes:
+    """Filtered corpus: each line holding a candidate window start, once, in
+    text order, with a trailing newline (printpositions, greptest.cu:353-398:
+    lines are delimited by '\n' (:375-390); a candidate on a '\n' byte or in an
+    already printed line prints nothing (:373); the reference's memchr past the
+    end (:387) is read as "the line ends at the end of the text")."""
+    t = bytes(np.asarray(text, dtype=np.uint8))
+    out = []
+    prev_end = -1
+    for o in sorted(set(int(x) for x in cand)):
+        if o <= prev_end or t[o:o + 1] == b"\n":
+            continue
+        start = t.rfind(b"\n", 0, o) + 1
+        end = t.find(b"\n", o)
+        if end < 0:
+            end = len(t)
+        out.append(t[start:end] + b"\n")
+        prev_end = end
+    return b"".join(out)
+
+
+def survivor_files(pat_bytes, pat_off, surv_bitmap):
+    """(filtered patterns, index) file contents: every surviving pattern on
+    its own line in id order, and its 0-based line number (filterPhrases,
+    ffbf.cpp:430-440 writes the line and lineNumber; ids count from 0 at
+    :404)."""
+    b = bytes(np.asarray(pat_bytes, dtype=np.uint8))
+    off = [int(x) for x in pat_off]
+    bits = np.asarray(surv_bitmap, dtype=np.uint32)
+    pats, idx = [], []
+    for i in range(len(off) - 1):
+        if (int(bits[i >> 5]) >> (i & 31)) & 1:
+            pats.append(b[off[i]:off[i + 1]] + b"\n")
+            idx.append(b"%d\n" % i)
+    return b"".join(pats), b"".join(idx)

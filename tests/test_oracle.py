@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import bits_of, load_golden, words_of
-from oracle import Oracle, brute_matches, filter_load_deltas, filter_save_deltas
+from oracle import Oracle, brute_matches, candidate_lines, filter_load_deltas, filter_save_deltas, survivor_files
 
 
 def rotl(x, r):
@@ -214,3 +214,25 @@ def _loop(S, w, r1, r2):
     for c in w:
         h1, h2 = rotl(h1, r1) ^ S[c], rotl(h2, r2) ^ S[c]
     return h1, h2
+
+
+def test_candidate_lines_hand_cases():
+    # lines: "ab" [0,2), "\n" 2, "cd" [3,5), "\n" 5, "" (6 is '\n'), "efg" [7,10) without '\n'
+    text = np.frombuffer(b"ab\ncd\n\nefg", dtype=np.uint8)
+    assert candidate_lines(text, []) == b""
+    assert candidate_lines(text, [0, 1]) == b"ab\n"           # two candidates, one line
+    assert candidate_lines(text, [2]) == b""                  # on a line break
+    assert candidate_lines(text, [6]) == b""                  # on an empty line's break
+    assert candidate_lines(text, [4, 3, 9]) == b"cd\nefg\n"    # order-free; last line ends at the end
+    assert candidate_lines(text, [1, 3, 8, 8]) == b"ab\ncd\nefg\n"
+    # a text without any line break is one line
+    assert candidate_lines(np.frombuffer(b"xyz", dtype=np.uint8), [2]) == b"xyz\n"
+
+
+def test_survivor_files_hand_cases():
+    b = np.frombuffer(b"alphabetagamma", dtype=np.uint8)
+    off = np.array([0, 5, 9, 14], dtype=np.uint64)
+    bits = np.array([0b101], dtype=np.uint32)
+    pats, idx = survivor_files(b, off, bits)
+    assert pats == b"alpha\ngamma\n" and idx == b"0\n2\n"
+    assert survivor_files(b, off, np.array([0], dtype=np.uint32)) == (b"", b"")
