@@ -2,11 +2,11 @@
 //
 // A server owns `slots` workers.  Each slot has its own copy of every filter
 // (pattern set, F and prefilters copied device to device; its own R and
-// survivor index), a batch object on its own stream, a pinned staging buffer
-// and a device text buffer of max_batch_bytes.  A free slot takes, in arrival
-// order, every pending request that still fits its text buffer and the
-// server's HBM budget, stages the texts, runs one batched scan
-// (ffbf_batch_run) and hands every request its own matches.  Requests that
+// survivor index), a batch object on its own stream and a device text buffer
+// of max_batch_bytes.  A free slot takes, in arrival order, every pending
+// request that still fits its text buffer and the server's HBM budget, copies
+// the texts into its buffer straight from the requests' memory, runs one
+// batched scan (ffbf_batch_run) and hands every request its own matches.  Requests that
 // arrive while slots are busy wait in the queue and are admitted by the next
 // slot to free up; two slots work on the device at once, so one batch's
 // copies and scan overlap another's.  A request's matches are those of
