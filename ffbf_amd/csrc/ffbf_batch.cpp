@@ -34,11 +34,13 @@ struct ffbf_batch {
     DevBuf<BatchText> d_texts;
     DevBuf<BatchChunk> d_chunks;
     // per chunk class (0: LDS prefilter, 1: prefilter in global memory), four
-    // words each: next chunk, number of chunks, index of the first, unused
+    // words each: next chunk, number of chunks, index of the first; word 3
+    // holds the number of texts (read by verification), word 7 is unused
     DevBuf<unsigned long long> ctl;
     DevBuf<unsigned long long> counters;  // C_CAND, C_MATCH
     unsigned long long *h_counters = nullptr;
-    DevBuf<Cand> cand;
+    DevBuf<Cand> cand;  // compacted candidate list
+    CandStage cstage;   // staging of both scan launches before compaction
     DevBuf<BatchMatchRec> matches;
     DevBuf<uint64_t> sort_keys;
     DevBuf<uint8_t> sort_temp;
@@ -117,8 +119,12 @@ int refresh_filters(ffbf_batch *b)
         a.R = c->R.p;
         a.pre = c->mode == MODE_FOLD ? c->fold.p : c->mode == MODE_GFOLD ? c->gfold.p : c->F.p;
         a.sbox = c->sbox.p;
-        a.cand = b->cand.p;
-        a.cand_cap = b->cand.cap;
+        // region slots: LDS-class launch [0, half), global-class launch [half, all)
+        const uint32_t half = b->cstage.n_regions / 2u;
+        if (lds)
+            b->cstage.fill(a, 0u, half);
+        else
+            b->cstage.fill(a, half, b->cstage.n_regions);
         a.counters = b->counters.p;
         a.W = c->params.window;
         a.k = c->params.k;
@@ -153,6 +159,7 @@ int enqueue_run(ffbf_batch *b, bool timed)
     HIP_TRY(launch_fill32((uint32_t *)b->ctl.p, 2u, 0u, b->stream));        // next chunk = 0, class 0
     HIP_TRY(launch_fill32((uint32_t *)(b->ctl.p + 4), 2u, 0u, b->stream));  // and class 1
     HIP_TRY(launch_fill32((uint32_t *)b->counters.p, 2u * C_COUNT, 0u, b->stream));
+    HIP_TRY(launch_fill32(b->cstage.counts.p, b->cstage.n_regions, 0u, b->stream));
     for (ffbf_ctx *c : b->filters) {
         HIP_TRY(launch_fill32(c->R.p, c->f_words, 0u, b->stream));
         HIP_TRY(launch_fill32(c->table.p, 2ull * c->table_size, 0u, b->stream));
@@ -174,6 +181,9 @@ int enqueue_run(ffbf_batch *b, bool timed)
         ba.ctl = b->ctl.p + 4;
         HIP_TRY(launch_scan_batch_global(ba, b->n_cu, b->stream));
     }
+    HIP_TRY(launch_cand_compact(b->cstage.raw.p, b->cstage.raw.cap, b->cstage.counts.p, b->cstage.offsets.p,
+                                b->cstage.n_regions, b->cstage.region, b->cand.p, b->cand.cap, b->counters.p, b->n_cu,
+                                b->stream));
     if (timed) HIP_TRY(hipEventRecord(b->e_scan1, b->stream));
     for (ffbf_ctx *c : b->filters) {
         PassArgs pa{};
@@ -202,6 +212,7 @@ int enqueue_run(ffbf_batch *b, bool timed)
     va.out = b->matches.p;
     va.out_cap = b->matches.cap;
     va.counters = b->counters.p;
+    va.ctl = b->ctl.p;
     HIP_TRY(launch_verify_batch(va, b->n_cu, b->stream));
     HIP_TRY(hipMemcpyAsync(b->h_counters, b->counters.p, C_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->stream));
@@ -303,6 +314,7 @@ int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch *
         RET_IF(b->ctl.ensure(8, "batch control"));
         RET_IF(b->counters.ensure(C_COUNT, "batch counters"));
         RET_IF(b->cand.ensure(1 << 16, "batch candidates"));
+        RET_IF(b->cstage.ensure(b->n_cu, b->cand.cap));
         RET_IF(b->matches.ensure(1 << 16, "batch matches"));
         // the filters' own streams may still be building
         for (ffbf_ctx *c : b->filters) HIP_TRY(hipStreamSynchronize(c->stream));
@@ -332,6 +344,7 @@ void ffbf_batch_destroy(ffbf_batch *b)
     b->ctl.release();
     b->counters.release();
     b->cand.release();
+    b->cstage.release();
     b->matches.release();
     b->sort_keys.release();
     b->sort_temp.release();
@@ -418,6 +431,7 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     if (b->cand.cap < want) {
         drop_graph(b);
         RET_IF(b->cand.ensure(std::max(want, 2 * b->cand.cap), "batch candidates"));
+        RET_IF(b->cstage.ensure(b->n_cu, b->cand.cap));
         RET_IF(refresh_filters(b));
     }
     if (n_texts) HIP_TRY(hipMemcpyAsync(b->d_texts.p, tx.data(), tx.size() * sizeof(BatchText), hipMemcpyHostToDevice, b->stream));
@@ -426,7 +440,7 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
                                b->stream));
     uint64_t n0 = 0;
     while (n0 < chunks.size() && cls(chunks[n0].filter) == 0u) n0++;
-    const unsigned long long ctl[8] = {0ull, n0, 0ull, 0ull, 0ull, chunks.size() - n0, n0, 0ull};
+    const unsigned long long ctl[8] = {0ull, n0, 0ull, n_texts, 0ull, chunks.size() - n0, n0, 0ull};
     HIP_TRY(hipMemcpyAsync(b->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->have_texts = true;
@@ -444,7 +458,10 @@ int ffbf_batch_run(ffbf_batch *b, uint64_t *n_matches)
         if (nc <= b->cand.cap && nm <= b->matches.cap) break;
         // grow and run again: every stage is a pure function of its inputs
         drop_graph(b);
-        if (nc > b->cand.cap) RET_IF(b->cand.ensure(std::max(2 * nc, 2 * b->cand.cap), "batch candidates"));
+        if (nc > b->cand.cap) {
+            RET_IF(b->cand.ensure(std::max(2 * nc, 2 * b->cand.cap), "batch candidates"));
+            RET_IF(b->cstage.ensure(b->n_cu, b->cand.cap));
+        }
         if (nm > b->matches.cap) RET_IF(b->matches.ensure(std::max(2 * nm, 2 * b->matches.cap), "batch matches"));
         RET_IF(refresh_filters(b));
         if (attempt == 2) return fail(FFBF_EDEVICE, "batch outputs kept overflowing");

@@ -110,27 +110,29 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         else
             drain<false, 1>(q, qn, a, s_pre, pend);
     }
-    if (loaded != 0xFFFFFFFFu) flush_pending(pend, a);
+    if (loaded != 0xFFFFFFFFu) finish_pending(pend, a);
 }
 
 // Exact verification of a batch: the candidate's text id selects the text
 // and, through it, the filter whose survivor index and patterns are used.
 __global__ __launch_bounds__(256) void k_verify_batch(BatchVerifyArgs v)
 {
-    const unsigned long long total = v.counters[C_CAND];
+    const unsigned long long total = v.counters[C_CVALID];
     const uint64_t count = total < v.cand_cap ? total : v.cand_cap;
+    const uint64_t n_texts = v.ctl[3];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < count; c += stride) {
         const Cand e = v.cand[c];
-        const uint32_t tid = (uint32_t)(e.offset >> kBatchOffsetBits);
+        const uint64_t tid = e.offset >> kBatchOffsetBits;
         const uint64_t off = e.offset & ((1ull << kBatchOffsetBits) - 1ull);
+        if (tid >= n_texts) continue;  // never for a recorded candidate
         const BatchText t = v.tdesc[tid];
         const VerifyArgs &f = v.filters[t.filter];
         verify_candidate(f, v.texts + t.start, t.n, off, e.h1, e.h2, [&](uint32_t id) {
             const unsigned long long s = atomicAdd(v.counters + C_MATCH, 1ull);
             if (s < v.out_cap) {
                 BatchMatchRec m;
-                m.text = tid;
+                m.text = (uint32_t)tid;
                 m.id = id;
                 m.offset = off;
                 v.out[s] = m;

@@ -200,12 +200,21 @@ uint32_t scan_block(ffbf_ctx *c)
     return 256u;
 }
 
-// Clears R and the candidate count before the first piece of a scan.
+int ensure_cand(ffbf_ctx *c, uint64_t n)
+{
+    if (n > c->cand.cap || !c->cand.p) RET_IF(c->cand.ensure(std::max(n, 2 * c->cand.cap), "candidates"));
+    RET_IF(c->cstage.ensure(c->n_cu, c->cand.cap));
+    return FFBF_OK;
+}
+
+// Clears R, the candidate list and its staging before the first piece of a scan.
 int enqueue_scan_begin(ffbf_ctx *c)
 {
     RET_IF(record(c, EV_S0));
     HIP_TRY(ffbf::launch_fill32(c->R.p, c->f_words, 0u, c->stream));
+    HIP_TRY(ffbf::launch_fill32(c->cstage.counts.p, c->cstage.n_regions, 0u, c->stream));
     RET_IF(clear_counters(c, ffbf::C_CAND, 1));
+    RET_IF(clear_counters(c, ffbf::C_SPILL, 2));  // and C_CVALID
     RET_IF(record(c, EV_S1));
     return FFBF_OK;
 }
@@ -222,8 +231,7 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
     a.F = c->F.p;
     a.R = c->R.p;
     a.sbox = c->sbox.p;
-    a.cand = c->cand.p;
-    a.cand_cap = c->cand.cap;
+    c->cstage.fill(a, 0u, c->cstage.n_regions);
     a.counters = c->counters.p;
     a.W = c->params.window;
     a.k = c->params.k;
@@ -239,6 +247,9 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
     a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
     HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
+    HIP_TRY(ffbf::launch_cand_compact(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
+                                      c->cstage.n_regions, c->cstage.region, c->cand.p, c->cand.cap, c->counters.p,
+                                      c->n_cu, c->stream));
     return FFBF_OK;
 }
 
@@ -312,7 +323,7 @@ int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n)
     c->text = dev_text;
     c->text_n = n;
     const uint64_t want = std::max<uint64_t>(65536, n >> 10);
-    if (c->cand.cap < want) RET_IF(c->cand.ensure(std::max(want, 2 * c->cand.cap), "candidates"));
+    RET_IF(ensure_cand(c, want));
     if (!c->matches.p) RET_IF(c->matches.ensure(65536, "matches"));
     c->scanned = c->passed = c->verified = false;
     c->sorted_valid = false;
@@ -325,7 +336,7 @@ int do_scan(ffbf_ctx *c)
     RET_IF(enqueue_scan(c));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
-        RET_IF(c->cand.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_CAND], 2 * c->cand.cap), "candidates"));
+        RET_IF(ensure_cand(c, 2 * c->h_counters[ffbf::C_CAND]));
         RET_IF(enqueue_scan(c));
         RET_IF(sync_counters(c));
     }
@@ -478,6 +489,7 @@ void ffbf_destroy(ffbf_ctx *c)
     c->chain_next.release();
     c->owned_text.release();
     c->cand.release();
+    c->cstage.release();
     c->matches.release();
     c->sort_keys.release();
     c->sort_temp.release();
@@ -631,7 +643,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
         // rerun the three stages with room for every candidate
-        RET_IF(c->cand.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_CAND], 2 * c->cand.cap), "candidates"));
+        RET_IF(ensure_cand(c, 2 * c->h_counters[ffbf::C_CAND]));
         RET_IF(enqueue_scan(c));
         RET_IF(enqueue_pass(c));
         RET_IF(enqueue_verify(c));

@@ -59,6 +59,54 @@ struct DevBuf {
 
 enum Ev { EV_B0, EV_B1, EV_S0, EV_S1, EV_S2, EV_P0, EV_P1, EV_V0, EV_V1, EV_N };
 
+// Where a scan launch stages its candidates before they are compacted into
+// the candidate list: one region per wave slot plus a spill area (ScanArgs),
+// the regions' fill counts and the compaction's offsets.  Sized from the
+// list's capacity; the counts are zeroed before every scan.
+struct CandStage {
+    DevBuf<Cand> raw;
+    DevBuf<uint32_t> counts;
+    DevBuf<uint64_t> offsets;
+    uint32_t n_regions = 0, region = 0;
+    uint64_t list_cap = 0;  // list capacity the layout was made for
+
+    int ensure(int n_cu, uint64_t cap)
+    {
+        if (raw.p && cap == list_cap) return FFBF_OK;
+        const uint64_t list_cap_new = cap;
+        const uint32_t nr = (uint32_t)n_cu * kRegionWavesPerCU;
+        // a launch uses at most half the slots: regions alone then hold a
+        // full list; the spill area takes half a list more
+        const uint32_t reg = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(4u, 2u * list_cap_new / nr), 4096u);
+        RET_IF(raw.ensure((uint64_t)nr * reg + list_cap_new / 2u + 1u, "candidate staging"));
+        RET_IF(counts.ensure(nr, "candidate region counts"));
+        RET_IF(offsets.ensure((uint64_t)nr + 2u, "candidate offsets"));
+        n_regions = nr;
+        region = reg;
+        list_cap = list_cap_new;
+        return FFBF_OK;
+    }
+    // the staging fields of a scan launch whose waves use slots [wave0, end)
+    void fill(ScanArgs &a, uint32_t wave0, uint32_t end) const
+    {
+        a.cand = raw.p;
+        a.cand_cap = raw.cap;
+        a.wave_counts = counts.p;
+        a.region = region;
+        a.wave0 = wave0;
+        a.region_end = end;
+        a.spill_base = (uint64_t)n_regions * region;
+    }
+    void release()
+    {
+        raw.release();
+        counts.release();
+        offsets.release();
+        n_regions = region = 0;
+        list_cap = 0;
+    }
+};
+
 uint32_t pow2_at_least(uint64_t n);
 
 }  // namespace ffbf
@@ -98,7 +146,8 @@ struct ffbf_ctx {
     const uint8_t *text = nullptr;
     uint64_t text_n = 0;
     ffbf::DevBuf<uint8_t> owned_text;
-    ffbf::DevBuf<ffbf::Cand> cand;
+    ffbf::DevBuf<ffbf::Cand> cand;  // compacted candidate list
+    ffbf::CandStage cstage;         // per-launch staging before compaction
     ffbf::DevBuf<ffbf::MatchRec> matches;
     ffbf::DevBuf<uint64_t> sort_keys;  // 2 x capacity
     ffbf::DevBuf<uint8_t> sort_temp;
@@ -133,6 +182,8 @@ int sync_counters(ffbf_ctx *c);
 int enqueue_scan_begin(ffbf_ctx *c);
 int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base);
 int do_scan(ffbf_ctx *c);
+// Grows the candidate list (and its staging) to hold at least n entries.
+int ensure_cand(ffbf_ctx *c, uint64_t n);
 // Text ingest (ffbf_ingest.cpp): host memory through the staging pipeline,
 // and release of the pipeline's buffers and stream.
 int scan_host(ffbf_ctx *c, const uint8_t *text, uint64_t n);

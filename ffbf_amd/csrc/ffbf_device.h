@@ -24,7 +24,18 @@ struct MatchRec {
 static_assert(sizeof(MatchRec) == 16, "MatchRec is 16 bytes");
 
 // Device counters (unsigned long long each).
-enum Counter : int { C_CAND = 0, C_SURV = 1, C_MATCH = 2, C_HASHED = 3, C_COUNT = 8 };
+// C_CAND: candidates found (beyond the list capacity when some were lost, so
+// that the caller grows the list and scans again); C_CVALID: leading entries
+// of the compacted list that hold candidates (what verification reads).
+enum Counter : int { C_CAND = 0, C_SURV = 1, C_MATCH = 2, C_HASHED = 3, C_SPILL = 4, C_CVALID = 5, C_COUNT = 8 };
+
+// Candidate staging of a scan launch: wave w of the launch appends to its own
+// region of `region` entries at (wave0 + w) * region of the raw candidate
+// buffer, with no atomics, and records how many it kept; a wave whose region
+// is full appends to the shared spill area after all regions (one counter
+// atomic per wave flush).  A compaction pass then appends every region and
+// the spill to the contiguous candidate list.
+constexpr uint32_t kRegionWavesPerCU = 64;  // region slots per CU: two launches of up to 32 waves
 
 // How the scan tests a window before the full forward probe.
 enum ScanMode : uint32_t {
@@ -52,6 +63,12 @@ struct ScanArgs {
     uint32_t pre_exact;  // prefilter is F itself: no further F probes
     uint32_t lds;        // prefilter is copied to LDS (batched scan: per filter)
     uint64_t off_base;   // added to every recorded offset (a text scanned in pieces)
+    // candidate staging (cand/cand_cap above are the raw buffer)
+    uint32_t *wave_counts;  // entries kept per region slot
+    uint32_t region;        // entries per wave region
+    uint32_t wave0;         // region slot of the launch's first wave
+    uint32_t region_end;    // slots [wave0, region_end) are the launch's; waves beyond only spill
+    uint64_t spill_base;    // first entry after all region slots
 };
 
 // Batched scan: many texts (concatenated in one device buffer), each routed
@@ -92,7 +109,7 @@ struct BatchArgs {
     const BatchChunk *chunks;
     const ScanArgs *filters;  // per filter; cand/counters are the batch's
     const uint32_t *sbox;
-    unsigned long long *ctl;  // [0] next chunk, [1] number of chunks, [2] index of the first
+    unsigned long long *ctl;  // [0] next chunk, [1] number of chunks, [2] index of the first, [3] number of texts
     uint32_t W;
 };
 
@@ -121,7 +138,8 @@ struct BatchVerifyArgs {
     const VerifyArgs *filters;  // per filter: pattern set and survivor index (text fields unused)
     BatchMatchRec *out;
     uint64_t out_cap;
-    unsigned long long *counters;  // C_CAND read, C_MATCH appended
+    unsigned long long *counters;  // C_CVALID read, C_MATCH appended
+    const unsigned long long *ctl;  // batch control words; [3] = number of texts
 };
 
 struct PassArgs {
@@ -150,6 +168,12 @@ hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_c
 hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
                          hipStream_t s);
 hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
+// Appends the regions and spill of the raw candidate buffer to the list
+// out[counters[C_CAND]..], updates C_CAND (beyond out_cap when anything was
+// lost) and zeroes the region counts and C_SPILL for the next launch.
+hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
+                               uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
+                               unsigned long long *counters, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
 
 hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,

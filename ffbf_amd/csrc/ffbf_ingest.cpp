@@ -226,7 +226,7 @@ int ingest(ffbf_ctx *c, Source &src, uint64_t size_hint)
     RET_IF(drain_copies(c));
     if (c->h_counters[C_CAND] > c->cand.cap) {
         // rescan the resident text with room for every candidate
-        RET_IF(c->cand.ensure(std::max<uint64_t>(2 * c->h_counters[C_CAND], 2 * c->cand.cap), "candidates"));
+        RET_IF(ensure_cand(c, 2 * c->h_counters[C_CAND]));
         RET_IF(do_scan(c));
     } else {
         c->n_cand = c->h_counters[C_CAND];

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
 // survivors' prefix pairs, then compares the whole pattern byte for byte.
 __global__ __launch_bounds__(256) void k_verify(VerifyArgs a)
 {
-    const unsigned long long total = a.counters[C_CAND];
+    const unsigned long long total = a.counters[C_CVALID];
     const uint64_t count = total < a.cand_cap ? total : a.cand_cap;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < count; c += stride) {
@@ -211,6 +211,90 @@ hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s)
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s)
 {
     hipLaunchKernelGGL(k_verify, dim3((uint32_t)(n_cu * 4)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+
+namespace {
+
+// Exclusive prefix of the region counts (one workgroup of 1024): offsets[w] =
+// list length before region w, offsets[n] = where the spill goes,
+// offsets[n+1] = spill entries kept.  Counts and the spill counter are
+// zeroed for the next launch; C_CAND becomes the new length, or exceeds
+// out_cap when entries were lost (the caller then grows and rescans).
+__global__ __launch_bounds__(1024) void k_cand_offsets(uint32_t *counts, uint64_t *offsets, uint32_t n, uint64_t spill_cap,
+                                                      uint64_t out_cap, unsigned long long *counters)
+{
+    __shared__ uint64_t s_part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t lo = t * per, hi = lo + per < n ? lo + per : n;
+    uint64_t sum = 0;
+    for (uint32_t i = lo; i < hi; i++) sum += counts[i];
+    s_part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint64_t v = t >= d ? s_part[t - d] : 0u;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    const uint64_t base = counters[C_CAND];
+    uint64_t run = base + (t ? s_part[t - 1] : 0u);
+    for (uint32_t i = lo; i < hi; i++) {
+        offsets[i] = run;
+        run += counts[i];
+        counts[i] = 0u;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const uint64_t end_regions = base + s_part[1023];
+        const uint64_t spill = counters[C_SPILL];
+        const uint64_t kept = spill < spill_cap ? spill : spill_cap;
+        offsets[n] = end_regions;
+        offsets[n + 1] = kept;
+        const uint64_t end = end_regions + spill;
+        counters[C_CAND] = (spill > spill_cap && end <= out_cap) ? out_cap + 1u : end;
+        counters[C_SPILL] = 0u;
+        // the list is whole up to the last entry written, if it was whole
+        // before this launch; a lost spill leaves a gap after it
+        const uint64_t written = end_regions + kept < out_cap ? end_regions + kept : out_cap;
+        if (counters[C_CVALID] == base) counters[C_CVALID] = written;
+    }
+}
+
+// Appends each region (one wave per region) and then the spill to out.
+__global__ __launch_bounds__(256) void k_cand_gather(const Cand *raw, const uint64_t *offsets, uint32_t n, uint32_t region,
+                                                     uint64_t spill_base, Cand *out, uint64_t out_cap)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t w = gw; w < n; w += nw) {
+        const uint64_t o = offsets[w], cnt = offsets[w + 1] - o;
+        for (uint64_t i = lane; i < cnt; i += 64u)
+            if (o + i < out_cap) out[o + i] = raw[(uint64_t)w * region + i];
+    }
+    const uint64_t so = offsets[n], sc = offsets[n + 1];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sc; i += stride)
+        if (so + i < out_cap) out[so + i] = raw[spill_base + i];
+}
+
+}  // namespace
+
+hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
+                               uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
+                               unsigned long long *counters, int n_cu, hipStream_t s)
+{
+    const uint64_t spill_base = (uint64_t)n_regions * region;
+    if (raw_cap < spill_base) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cand_offsets, dim3(1), dim3(1024), 0, s, wave_counts, offsets, n_regions, raw_cap - spill_base,
+                       out_cap, counters);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cand_gather, dim3((uint32_t)(n_cu * 2)), dim3(256), 0, s, raw, offsets, n_regions, region,
+                       spill_base, out, out_cap);
     return hipGetLastError();
 }
 

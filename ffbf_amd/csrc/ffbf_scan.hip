@@ -80,7 +80,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
         wave_sync();
         drain<LDSPRE, P>(q, qn, a, s_pre, pend);
     }
-    flush_pending(pend, a);
+    finish_pending(pend, a);
 }
 
 template <int WT, bool LDSPRE, int BLOCK, int P>

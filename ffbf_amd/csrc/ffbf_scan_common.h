@@ -146,21 +146,60 @@ static __device__ __forceinline__ void hash_unit_generic(const uint8_t *base, ui
     }
 }
 
-// A lane's candidate not yet appended to the global list.  The list's counter
-// is a device-wide atomic that every appending wave hits, so appends go out
-// in bulk: a wave flushes all its held candidates (one atomic) only when a
-// new hit lands on a lane that still holds one, and at the end of its work.
+// A lane's candidate not yet appended.  Appends go out in bulk: a wave
+// flushes all its held candidates when a new hit lands on a lane that still
+// holds one, and at the end of its work.  A flush writes into the wave's own
+// region of the raw candidate buffer (no atomics); only a wave whose region
+// is full takes slots in the shared spill area, with one counter atomic (a
+// device-wide atomic on one address that every CU would otherwise hit).
 struct PendingCand {
     uint64_t off = 0;
     uint32_t h1 = 0, h2 = 0;
     bool valid = false;
+    uint32_t kept = 0;  // wave-uniform: entries written to the wave's region
 };
+
+static __device__ __forceinline__ uint32_t region_slot(const ScanArgs &a)
+{
+    return a.wave0 + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+}
 
 // Called by the whole wave.
 static __device__ __forceinline__ void flush_pending(PendingCand &pc, const ScanArgs &a)
 {
-    append_cand(pc.valid, pc.off, pc.h1, pc.h2, a.cand, a.cand_cap, a.counters + C_CAND);
+    const uint64_t hb = __ballot(pc.valid);
+    if (!hb) return;
+    const uint32_t n = (uint32_t)__builtin_popcountll(hb);
+    const uint32_t slot_id = region_slot(a);
+    uint64_t at;
+    if (slot_id < a.region_end && pc.kept + n <= a.region) {
+        at = (uint64_t)slot_id * a.region + pc.kept;
+        pc.kept += n;
+    } else {
+        const int leader = __builtin_ctzll(hb);
+        unsigned long long first = 0;
+        if (lane_id() == leader) first = atomicAdd(a.counters + C_SPILL, (unsigned long long)n);
+        at = a.spill_base + readlane64(first, leader);
+    }
+    if (pc.valid) {
+        const uint64_t e = at + lanes_below(hb);
+        if (e < a.cand_cap) {
+            Cand c;
+            c.offset = pc.off;
+            c.h1 = pc.h1;
+            c.h2 = pc.h2;
+            a.cand[e] = c;
+        }
+    }
     pc.valid = false;
+}
+
+// End of a wave's work: flush, then record the region's fill for the
+// compaction pass (counts start at zero, so an empty region writes nothing).
+static __device__ __forceinline__ void finish_pending(PendingCand &pc, const ScanArgs &a)
+{
+    flush_pending(pc, a);
+    if (pc.kept && lane_id() == 0) a.wave_counts[region_slot(a)] = pc.kept;  // kept > 0: slot < region_end
 }
 
 // Called by the whole wave.
