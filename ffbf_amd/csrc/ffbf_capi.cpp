@@ -608,7 +608,12 @@ int ffbf_scan(ffbf_ctx *c, const uint8_t *text, uint64_t n)
 {
     RET_IF(check_ctx(c));
     if (!text && n) return fail(FFBF_EINVAL, "null text");
-    return scan_host(c, text, n);
+    if (n > c->ingest_chunk) return scan_host(c, text, n);
+    // one piece: nothing to overlap
+    RET_IF(c->owned_text.ensure(std::max<uint64_t>(n, 1), "text"));
+    if (n) HIP_TRY(hipMemcpyAsync(c->owned_text.p, text, n, hipMemcpyHostToDevice, c->stream));
+    RET_IF(set_text(c, c->owned_text.p, n));
+    return do_scan(c);
 }
 
 int ffbf_pattern_pass(ffbf_ctx *c, uint64_t *n_survivors)
@@ -685,9 +690,16 @@ int ffbf_match_text(ffbf_ctx *c, const uint8_t *text, uint64_t n, ffbf_match **o
     *n_out = 0;
     if (!text && n) return fail(FFBF_EINVAL, "null text");
     uint64_t nm = 0;
-    RET_IF(scan_host(c, text, n));
-    RET_IF(ffbf_pattern_pass(c, nullptr));
-    RET_IF(ffbf_verify(c, &nm));
+    if (n <= c->ingest_chunk) {
+        // one piece: a single copy and the three stages back to back, one sync
+        RET_IF(c->owned_text.ensure(std::max<uint64_t>(n, 1), "text"));
+        if (n) HIP_TRY(hipMemcpyAsync(c->owned_text.p, text, n, hipMemcpyHostToDevice, c->stream));
+        RET_IF(ffbf_run_device(c, c->owned_text.p, n, &nm));
+    } else {
+        RET_IF(scan_host(c, text, n));
+        RET_IF(ffbf_pattern_pass(c, nullptr));
+        RET_IF(ffbf_verify(c, &nm));
+    }
     RET_IF(load_sorted_matches(c));
     ffbf_match *m = (ffbf_match *)std::malloc(std::max<uint64_t>(nm, 1) * sizeof(ffbf_match));
     if (!m) return fail(FFBF_ENOMEM, "match list allocation");

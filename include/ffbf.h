@@ -108,10 +108,11 @@ int ffbf_set_patterns_lines(ffbf_ctx *ctx, const char *buf, uint64_t len);
 int ffbf_build(ffbf_ctx *ctx);
 
 /* Scans a host text: clears R and scans every offset in [0, n-W]
- * (executeGrepOverlap, greptest.cu:505-576).  The text moves into device
- * memory owned by the context through pinned staging buffers, piece by
- * piece; each piece's copy overlaps the scan of the windows already complete
- * (a window straddling two pieces is scanned once both have landed). */
+ * (executeGrepOverlap, greptest.cu:505-576).  A text longer than one ingest
+ * piece (ffbf_set_ingest) moves into device memory owned by the context
+ * through pinned staging buffers, piece by piece; each piece's copy overlaps
+ * the scan of the windows already complete (a window straddling two pieces
+ * is scanned once both have landed).  A shorter text is copied in one go. */
 int ffbf_scan(ffbf_ctx *ctx, const uint8_t *text, uint64_t n);
 
 /* The same for everything readable from a file descriptor, from its current
