@@ -31,6 +31,12 @@ struct ffbf_batch {
 
     DevBuf<ScanArgs> d_scan;
     DevBuf<VerifyArgs> d_ver;
+    DevBuf<PassArgs> d_pass;          // per filter, for the one-launch pattern pass
+    DevBuf<uint64_t> d_wave_start;    // first 64-id wave of each filter's pass (nf+1)
+    uint64_t pass_waves = 0;
+    DevBuf<uint32_t> d_active;        // per filter: some text of the batch is routed to it
+    DevBuf<ZeroSpan> d_zero;          // everything a run clears first, in one launch
+    uint32_t n_zero = 0;
     DevBuf<BatchText> d_texts;
     DevBuf<BatchChunk> d_chunks;
     // per chunk class (0: LDS prefilter, 1: prefilter in global memory), four
@@ -147,8 +153,55 @@ int refresh_filters(ffbf_batch *b)
     }
     RET_IF(b->d_scan.ensure(nf, "batch filter descriptors"));
     RET_IF(b->d_ver.ensure(nf, "batch verify descriptors"));
+    // pattern passes: descriptors and each filter's first wave
+    std::vector<PassArgs> pa(nf);
+    std::vector<uint64_t> ws(nf + 1, 0);
+    for (uint32_t f = 0; f < nf; f++) {
+        const ffbf_ctx *c = b->filters[f];
+        PassArgs &p = pa[f];
+        p = PassArgs{};
+        p.pat_hash = c->pat_hash.p;
+        p.off = c->off.p;
+        p.n = c->n_pat;
+        p.W = c->params.window;
+        p.k = c->params.k;
+        p.mask = c->mask;
+        p.R = c->R.p;
+        p.bitmap = c->bitmap.p;
+        p.surv_ids = c->surv_ids.p;
+        p.counters = c->counters.p;
+        p.table = c->table.p;
+        p.slot_chain = c->table.p + c->table_size;
+        p.chain_next = c->chain_next.p;
+        p.table_mask = c->table_size - 1u;
+        ws[f + 1] = ws[f] + (c->n_pat + 63u) / 64u;
+    }
+    b->pass_waves = ws[nf];
+    // what a run clears before its scans, as spans of at most 65536 words
+    std::vector<ZeroSpan> zs;
+    auto zero = [&](uint32_t *p, uint64_t words) {
+        for (uint64_t o = 0; o < words; o += 65536u)
+            zs.push_back(ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(65536u, words - o)});
+    };
+    zero((uint32_t *)b->ctl.p, 2u);        // next chunk, class 0
+    zero((uint32_t *)(b->ctl.p + 4), 2u);  // next chunk, class 1
+    zero((uint32_t *)b->counters.p, 2u * C_COUNT);
+    zero(b->cstage.counts.p, b->cstage.n_regions);
+    for (ffbf_ctx *c : b->filters) {
+        zero(c->R.p, c->f_words);
+        zero(c->table.p, 2ull * c->table_size);
+        zero((uint32_t *)(c->counters.p + C_SURV), 2u);
+    }
+    b->n_zero = (uint32_t)zs.size();
+    RET_IF(b->d_pass.ensure(nf, "batch pass descriptors"));
+    RET_IF(b->d_wave_start.ensure(nf + 1, "batch pass waves"));
+    RET_IF(b->d_zero.ensure(zs.size(), "batch clear list"));
+    RET_IF(b->d_active.ensure(nf, "batch filter flags"));
     HIP_TRY(hipMemcpyAsync(b->d_scan.p, sa.data(), nf * sizeof(ScanArgs), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(b->d_ver.p, va.data(), nf * sizeof(VerifyArgs), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_pass.p, pa.data(), nf * sizeof(PassArgs), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_wave_start.p, ws.data(), (nf + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_zero.p, zs.data(), zs.size() * sizeof(ZeroSpan), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     return FFBF_OK;
 }
@@ -156,15 +209,9 @@ int refresh_filters(ffbf_batch *b)
 // The whole batch run on b->stream (capturable: kernels and one D2H copy).
 int enqueue_run(ffbf_batch *b, bool timed)
 {
-    HIP_TRY(launch_fill32((uint32_t *)b->ctl.p, 2u, 0u, b->stream));        // next chunk = 0, class 0
-    HIP_TRY(launch_fill32((uint32_t *)(b->ctl.p + 4), 2u, 0u, b->stream));  // and class 1
-    HIP_TRY(launch_fill32((uint32_t *)b->counters.p, 2u * C_COUNT, 0u, b->stream));
-    HIP_TRY(launch_fill32(b->cstage.counts.p, b->cstage.n_regions, 0u, b->stream));
-    for (ffbf_ctx *c : b->filters) {
-        HIP_TRY(launch_fill32(c->R.p, c->f_words, 0u, b->stream));
-        HIP_TRY(launch_fill32(c->table.p, 2ull * c->table_size, 0u, b->stream));
-        HIP_TRY(launch_fill32((uint32_t *)(c->counters.p + C_SURV), 2u, 0u, b->stream));
-    }
+    // chunk counters, batch counters, candidate region counts, and every
+    // filter's R, survivor index and survivor count
+    HIP_TRY(launch_zero_spans(b->d_zero.p, b->n_zero, b->n_cu, b->stream));
     if (timed) HIP_TRY(hipEventRecord(b->e_scan0, b->stream));
     BatchArgs ba{};
     ba.texts = b->texts;
@@ -185,24 +232,8 @@ int enqueue_run(ffbf_batch *b, bool timed)
                                 b->cstage.n_regions, b->cstage.region, b->cand.p, b->cand.cap, b->counters.p, b->n_cu,
                                 b->stream));
     if (timed) HIP_TRY(hipEventRecord(b->e_scan1, b->stream));
-    for (ffbf_ctx *c : b->filters) {
-        PassArgs pa{};
-        pa.pat_hash = c->pat_hash.p;
-        pa.off = c->off.p;
-        pa.n = c->n_pat;
-        pa.W = c->params.window;
-        pa.k = c->params.k;
-        pa.mask = c->mask;
-        pa.R = c->R.p;
-        pa.bitmap = c->bitmap.p;
-        pa.surv_ids = c->surv_ids.p;
-        pa.counters = c->counters.p;
-        pa.table = c->table.p;
-        pa.slot_chain = c->table.p + c->table_size;
-        pa.chain_next = c->chain_next.p;
-        pa.table_mask = c->table_size - 1u;
-        HIP_TRY(launch_pattern_pass(pa, c->n_cu, b->stream));
-    }
+    HIP_TRY(launch_pattern_pass_multi(b->d_pass.p, b->d_wave_start.p, (uint32_t)b->filters.size(), b->d_active.p,
+                                      b->pass_waves, b->n_cu, b->stream));
     BatchVerifyArgs va{};
     va.texts = b->texts;
     va.tdesc = b->d_texts.p;
@@ -345,6 +376,10 @@ void ffbf_batch_destroy(ffbf_batch *b)
     b->counters.release();
     b->cand.release();
     b->cstage.release();
+    b->d_pass.release();
+    b->d_wave_start.release();
+    b->d_active.release();
+    b->d_zero.release();
     b->matches.release();
     b->sort_keys.release();
     b->sort_temp.release();
@@ -438,6 +473,10 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     if (!chunks.empty())
         HIP_TRY(hipMemcpyAsync(b->d_chunks.p, chunks.data(), chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice,
                                b->stream));
+    std::vector<uint32_t> active(b->filters.size(), 0u);
+    for (const BatchChunk &ch : chunks) active[ch.filter] = 1u;  // texts with windows only
+    HIP_TRY(hipMemcpyAsync(b->d_active.p, active.data(), active.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           b->stream));
     uint64_t n0 = 0;
     while (n0 < chunks.size() && cls(chunks[n0].filter) == 0u) n0++;
     const unsigned long long ctl[8] = {0ull, n0, 0ull, n_texts, 0ull, chunks.size() - n0, n0, 0ull};

@@ -157,6 +157,12 @@ struct PassArgs {
     uint32_t table_mask;
 };
 
+// A run of u32 words to clear (at most 65536).
+struct ZeroSpan {
+    uint32_t *p;
+    uint32_t words;
+};
+
 // Launchers (hipError_t of the launch).
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
@@ -168,6 +174,12 @@ hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_c
 hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
                          hipStream_t s);
 hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
+// Pattern passes of nf filters in one launch (wave_start[nf+1], device
+// arrays); filters whose active flag is 0 get an empty survivor set.
+hipError_t launch_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start, uint32_t nf,
+                                     const uint32_t *active, uint64_t total_waves, int n_cu, hipStream_t s);
+// Clears every span of the list in one launch.
+hipError_t launch_zero_spans(const ZeroSpan *spans, uint32_t n, int n_cu, hipStream_t s);
 // Appends the regions and spill of the raw candidate buffer to the list
 // out[counters[C_CAND]..], updates C_CAND (beyond out_cap when anything was
 // lost) and zeroes the region counts and C_SPILL for the next launch.

@@ -82,55 +82,96 @@ __global__ void k_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint
 }
 
 // Pattern pass (filterPatterns, greptest.cu:253-290): each pattern's prefix
-// hash pair against R.  Each wave owns 64 consecutive ids, so its ballot is
-// two whole bitmap words; survivors are compacted with a wave prefix count
-// and indexed by prefix-hash key for verification.
-__global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
+// hash pair against R.  A wave owns 64 consecutive ids (i0 a multiple of 64),
+// so its ballot is two whole bitmap words; survivors are compacted with a
+// wave prefix count and indexed by prefix-hash key for verification.
+__device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
 {
     const int lane = lane_id();
+    const uint64_t n_words = (a.n + 31u) / 32u;
+    const uint64_t i = i0 + (uint64_t)lane;
+    bool ok = false;
+    uint2 h = make_uint2(0u, 0u);
+    if (i < a.n) {
+        ok = a.off[i + 1] - a.off[i] >= a.W;
+        h = a.pat_hash[i];
+        uint32_t words[5], idx[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            idx[j] = rev_idx(j, h.x, h.y) & a.mask;
+            words[j] = (ok && j < (int)a.k) ? a.R[idx[j] >> 5] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; j++) ok = ok && ((words[j] >> (idx[j] & 31u)) & 1u);
+    }
+    const uint64_t bal = __ballot(ok);
+    const uint64_t w = i0 >> 5;
+    if (lane == 0 && w < n_words) a.bitmap[w] = (uint32_t)bal;
+    if (lane == 32 && w + 1 < n_words) a.bitmap[w + 1] = (uint32_t)(bal >> 32);
+    if (!bal) return;
+    const int leader = __builtin_ctzll(bal);
+    unsigned long long first = 0;
+    if (lane == leader) first = atomicAdd(a.counters + C_SURV, (unsigned long long)__builtin_popcountll(bal));
+    first = readlane64(first, leader);
+    if (!ok) return;
+    a.surv_ids[first + lanes_below(bal)] = (uint32_t)i;
+    // index by key: the first id to claim a slot represents the key, the
+    // rest chain from the slot
+    uint32_t slot = key_slot(h.x, h.y) & a.table_mask;
+    for (;;) {
+        uint32_t cur = atomicCAS(&a.table[slot], 0u, (uint32_t)i + 1u);
+        if (cur == 0u) break;
+        const uint2 ch = a.pat_hash[cur - 1u];
+        if (ch.x == h.x && ch.y == h.y) {
+            a.chain_next[i] = atomicExch(&a.slot_chain[slot], (uint32_t)i + 1u);
+            break;
+        }
+        slot = (slot + 1u) & a.table_mask;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
+{
     const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t n_words = (a.n + 31u) / 32u;
-    for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) {
-        const uint64_t i = i0 + (uint64_t)lane;
-        bool ok = false;
-        uint2 h = make_uint2(0u, 0u);
-        if (i < a.n) {
-            ok = a.off[i + 1] - a.off[i] >= a.W;
-            h = a.pat_hash[i];
-            uint32_t words[5], idx[5];
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-                idx[j] = rev_idx(j, h.x, h.y) & a.mask;
-                words[j] = (ok && j < (int)a.k) ? a.R[idx[j] >> 5] : 0xFFFFFFFFu;
-            }
-#pragma unroll
-            for (int j = 0; j < 5; j++) ok = ok && ((words[j] >> (idx[j] & 31u)) & 1u);
+    for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) pass_wave(a, i0);
+}
+
+// The pattern passes of several filters in one launch: wave_start[f] is the
+// first global 64-id wave of filter f (prefix over ceil(n_f/64)).  A filter
+// whose `active` flag is clear (no text of the batch routed to it) has no
+// survivors: its bitmap words are cleared and nothing is indexed.
+__global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start,
+                                                            uint32_t nf, const uint32_t *active)
+{
+    const uint64_t total = wave_start[nf];
+    const uint64_t gw0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t stride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t gw = gw0; gw < total; gw += stride) {
+        uint32_t lo = 0, hi = nf;  // last f with wave_start[f] <= gw
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (wave_start[mid] <= gw) lo = mid;
+            else hi = mid;
         }
-        const uint64_t bal = __ballot(ok);
-        const uint64_t w = i0 >> 5;
-        if (lane == 0 && w < n_words) a.bitmap[w] = (uint32_t)bal;
-        if (lane == 32 && w + 1 < n_words) a.bitmap[w + 1] = (uint32_t)(bal >> 32);
-        if (!bal) continue;
-        const int leader = __builtin_ctzll(bal);
-        unsigned long long first = 0;
-        if (lane == leader) first = atomicAdd(a.counters + C_SURV, (unsigned long long)__builtin_popcountll(bal));
-        first = readlane64(first, leader);
-        if (!ok) continue;
-        a.surv_ids[first + lanes_below(bal)] = (uint32_t)i;
-        // index by key: the first id to claim a slot represents the key, the
-        // rest chain from the slot
-        uint32_t slot = key_slot(h.x, h.y) & a.table_mask;
-        for (;;) {
-            uint32_t cur = atomicCAS(&a.table[slot], 0u, (uint32_t)i + 1u);
-            if (cur == 0u) break;
-            const uint2 ch = a.pat_hash[cur - 1u];
-            if (ch.x == h.x && ch.y == h.y) {
-                a.chain_next[i] = atomicExch(&a.slot_chain[slot], (uint32_t)i + 1u);
-                break;
-            }
-            slot = (slot + 1u) & a.table_mask;
+        const PassArgs &a = filters[lo];
+        const uint64_t i0 = (gw - wave_start[lo]) << 6;
+        if (active[lo]) {
+            pass_wave(a, i0);
+        } else {
+            const uint64_t n_words = (a.n + 31u) / 32u, w = i0 >> 5;
+            const int lane = lane_id();
+            if (lane < 2 && w + (uint64_t)lane < n_words) a.bitmap[w + (uint64_t)lane] = 0u;
         }
+    }
+}
+
+// Zeroes a list of spans of u32 words, each at most 65536 words.
+__global__ __launch_bounds__(256) void k_zero_spans(const ZeroSpan *spans, uint32_t n)
+{
+    for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
+        const ZeroSpan z = spans[c];
+        for (uint32_t i = threadIdx.x; i < z.words; i += blockDim.x) z.p[i] = 0u;
     }
 }
 
@@ -205,6 +246,25 @@ hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s)
     const uint64_t cap = (uint64_t)n_cu * 8u;
     if (blocks > cap) blocks = cap;
     hipLaunchKernelGGL(k_pattern_pass, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start, uint32_t nf,
+                                     const uint32_t *active, uint64_t total_waves, int n_cu, hipStream_t s)
+{
+    if (total_waves == 0) return hipSuccess;
+    uint64_t blocks = (total_waves + 3u) / 4u;
+    const uint64_t cap = (uint64_t)n_cu * 8u;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_pattern_pass_multi, dim3((uint32_t)blocks), dim3(256), 0, s, filters, wave_start, nf, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_spans(const ZeroSpan *spans, uint32_t n, int n_cu, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = n < (uint32_t)n_cu * 4u ? n : (uint32_t)n_cu * 4u;
+    hipLaunchKernelGGL(k_zero_spans, dim3(blocks), dim3(256), 0, s, spans, n);
     return hipGetLastError();
 }
 
