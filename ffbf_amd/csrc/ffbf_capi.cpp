@@ -207,14 +207,42 @@ int ensure_cand(ffbf_ctx *c, uint64_t n)
     return FFBF_OK;
 }
 
-// Clears R, the candidate list and its staging before the first piece of a scan.
+// The spans a scan clears: R, the candidate region counts, the scan's
+// counters (C_CAND, C_SURV, C_MATCH, C_SPILL, C_CVALID) and the survivor
+// index, which the pattern pass fills.
+int build_zero_spans(ffbf_ctx *c)
+{
+    if (c->d_zero.p && c->zero_key[0] == c->cstage.counts.p && c->zero_key[1] == c->table.p &&
+        c->zero_table_size == c->table_size)
+        return FFBF_OK;
+    std::vector<ffbf::ZeroSpan> zs;
+    auto zero = [&](uint32_t *p, uint64_t words) {
+        for (uint64_t o = 0; o < words; o += 65536u)
+            zs.push_back(ffbf::ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(65536u, words - o)});
+    };
+    zero(c->R.p, c->f_words);
+    zero(c->cstage.counts.p, c->cstage.n_regions);
+    zero((uint32_t *)(c->counters.p + ffbf::C_CAND), 6u);   // C_CAND, C_SURV, C_MATCH
+    zero((uint32_t *)(c->counters.p + ffbf::C_SPILL), 4u);  // C_SPILL, C_CVALID
+    if (c->table.p) zero(c->table.p, 2ull * c->table_size);
+    RET_IF(c->d_zero.ensure(zs.size(), "clear list"));
+    HIP_TRY(hipMemcpyAsync(c->d_zero.p, zs.data(), zs.size() * sizeof(ffbf::ZeroSpan), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->n_zero = (uint32_t)zs.size();
+    c->zero_key[0] = c->cstage.counts.p;
+    c->zero_key[1] = c->table.p;
+    c->zero_table_size = c->table_size;
+    return FFBF_OK;
+}
+
+// Clears R, the candidate list and its staging, the scan's counters and the
+// survivor index before the first piece of a scan (one launch).
 int enqueue_scan_begin(ffbf_ctx *c)
 {
+    RET_IF(build_zero_spans(c));
     RET_IF(record(c, EV_S0));
-    HIP_TRY(ffbf::launch_fill32(c->R.p, c->f_words, 0u, c->stream));
-    HIP_TRY(ffbf::launch_fill32(c->cstage.counts.p, c->cstage.n_regions, 0u, c->stream));
-    RET_IF(clear_counters(c, ffbf::C_CAND, 1));
-    RET_IF(clear_counters(c, ffbf::C_SPILL, 2));  // and C_CVALID
+    HIP_TRY(ffbf::launch_zero_spans(c->d_zero.p, c->n_zero, c->n_cu, c->stream));
     RET_IF(record(c, EV_S1));
     return FFBF_OK;
 }
@@ -261,11 +289,15 @@ int enqueue_scan(ffbf_ctx *c)
     return FFBF_OK;
 }
 
-int enqueue_pass(ffbf_ctx *c)
+// fresh: the scan just cleared the survivor index and count (no earlier
+// pass since), so the pass need not clear them again.
+int enqueue_pass(ffbf_ctx *c, bool fresh)
 {
     RET_IF(record(c, EV_P0));
-    HIP_TRY(ffbf::launch_fill32(c->table.p, 2ull * c->table_size, 0u, c->stream));
-    RET_IF(clear_counters(c, ffbf::C_SURV, 1));
+    if (!fresh) {
+        HIP_TRY(ffbf::launch_fill32(c->table.p, 2ull * c->table_size, 0u, c->stream));
+        RET_IF(clear_counters(c, ffbf::C_SURV, 1));
+    }
     ffbf::PassArgs a{};
     a.pat_hash = c->pat_hash.p;
     a.off = c->off.p;
@@ -286,10 +318,10 @@ int enqueue_pass(ffbf_ctx *c)
     return FFBF_OK;
 }
 
-int enqueue_verify(ffbf_ctx *c)
+int enqueue_verify(ffbf_ctx *c, bool fresh)
 {
     RET_IF(record(c, EV_V0));
-    RET_IF(clear_counters(c, ffbf::C_MATCH, 1));
+    if (!fresh) RET_IF(clear_counters(c, ffbf::C_MATCH, 1));
     ffbf::VerifyArgs a{};
     a.text = c->text;
     a.n = c->text_n;
@@ -367,11 +399,11 @@ int sort_on_device(ffbf_ctx *c)
 
 int do_verify(ffbf_ctx *c)
 {
-    RET_IF(enqueue_verify(c));
+    RET_IF(enqueue_verify(c, false));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
         RET_IF(c->matches.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_MATCH], 2 * c->matches.cap), "matches"));
-        RET_IF(enqueue_verify(c));
+        RET_IF(enqueue_verify(c, false));
         RET_IF(sync_counters(c));
     }
     c->n_match = c->h_counters[ffbf::C_MATCH];
@@ -490,6 +522,7 @@ void ffbf_destroy(ffbf_ctx *c)
     c->owned_text.release();
     c->cand.release();
     c->cstage.release();
+    c->d_zero.release();
     c->matches.release();
     c->sort_keys.release();
     c->sort_temp.release();
@@ -620,7 +653,7 @@ int ffbf_pattern_pass(ffbf_ctx *c, uint64_t *n_survivors)
 {
     RET_IF(check_ctx(c));
     if (!c->scanned) return fail(FFBF_ESTATE, "pattern pass before scan");
-    RET_IF(enqueue_pass(c));
+    RET_IF(enqueue_pass(c, false));
     RET_IF(sync_counters(c));
     c->n_surv = c->h_counters[ffbf::C_SURV];
     c->passed = true;
@@ -643,15 +676,15 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     RET_IF(check_ctx(c));
     RET_IF(set_text(c, dev_text, n));
     RET_IF(enqueue_scan(c));
-    RET_IF(enqueue_pass(c));
-    RET_IF(enqueue_verify(c));
+    RET_IF(enqueue_pass(c, true));
+    RET_IF(enqueue_verify(c, true));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
         // rerun the three stages with room for every candidate
         RET_IF(ensure_cand(c, 2 * c->h_counters[ffbf::C_CAND]));
         RET_IF(enqueue_scan(c));
-        RET_IF(enqueue_pass(c));
-        RET_IF(enqueue_verify(c));
+        RET_IF(enqueue_pass(c, true));
+        RET_IF(enqueue_verify(c, true));
         RET_IF(sync_counters(c));
     }
     c->n_cand = c->h_counters[ffbf::C_CAND];
@@ -659,7 +692,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     c->scanned = c->passed = true;
     if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
         RET_IF(c->matches.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_MATCH], 2 * c->matches.cap), "matches"));
-        RET_IF(enqueue_verify(c));
+        RET_IF(enqueue_verify(c, false));
         RET_IF(sync_counters(c));
     }
     c->n_match = c->h_counters[ffbf::C_MATCH];

@@ -148,6 +148,12 @@ struct ffbf_ctx {
     ffbf::DevBuf<uint8_t> owned_text;
     ffbf::DevBuf<ffbf::Cand> cand;  // compacted candidate list
     ffbf::CandStage cstage;         // per-launch staging before compaction
+    // everything a scan clears first (R, region counts, scan counters,
+    // survivor index), cleared in one launch; rebuilt when a buffer moves
+    ffbf::DevBuf<ffbf::ZeroSpan> d_zero;
+    uint32_t n_zero = 0;
+    const void *zero_key[2] = {nullptr, nullptr};  // region counts and index the list was made for
+    uint32_t zero_table_size = 0;
     ffbf::DevBuf<ffbf::MatchRec> matches;
     ffbf::DevBuf<uint64_t> sort_keys;  // 2 x capacity
     ffbf::DevBuf<uint8_t> sort_temp;
@@ -192,6 +198,6 @@ void ingest_release(ffbf_ctx *c);
 // The stream a batch object queues its work on (ffbf_batch.cpp).
 hipStream_t batch_stream(ffbf_batch *b);
 // Stage launchers shared with the batch path (ffbf_capi.cpp).
-int enqueue_pass(ffbf_ctx *c);
+int enqueue_pass(ffbf_ctx *c, bool fresh);
 int use_device(ffbf_ctx *c);
 }  // namespace ffbf
