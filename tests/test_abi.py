@@ -111,3 +111,20 @@ def test_sbox_seeded():
     assert s1.shape == (256,) and not np.array_equal(s1, s2)
     assert np.array_equal(s1, fa.sbox_from_seed(1))
     assert len(np.unique(s1)) == 256
+
+
+def test_lognormal_sizes_deterministic_clipped_and_shaped():
+    a = fa.lognormal_sizes(41, 20000, 1024 * 1024.0, 1.0, 4096, 64 << 20)
+    b = fa.lognormal_sizes(41, 20000, 1024 * 1024.0, 1.0, 4096, 64 << 20)
+    assert np.array_equal(a, b)  # same seed, same list
+    assert not np.array_equal(a, fa.lognormal_sizes(42, 20000, 1024 * 1024.0, 1.0, 4096, 64 << 20))
+    assert a.min() >= 4096 and a.max() <= 64 << 20
+    # log-sizes: median ~ log(median), spread ~ sigma (clipping trims little at sigma 1)
+    la = np.log(a.astype(np.float64))
+    assert abs(np.median(la) - np.log(1024 * 1024.0)) < 0.05
+    assert 0.9 < np.std(la) < 1.1
+    # sigma 0 gives the median itself; a narrow clip range clamps everything
+    assert np.all(fa.lognormal_sizes(1, 100, 5000.0, 0.0, 1, 1 << 20) == 5000)
+    assert np.all(fa.lognormal_sizes(1, 100, 5000.0, 2.0, 777, 777) == 777)
+    with pytest.raises(fa.FFBFError):
+        fa.lognormal_sizes(1, 10, 5000.0, 1.0, 10, 5)  # lo > hi
