@@ -177,11 +177,11 @@ int refresh_filters(ffbf_batch *b)
         ws[f + 1] = ws[f] + (c->n_pat + 63u) / 64u;
     }
     b->pass_waves = ws[nf];
-    // what a run clears before its scans, as spans of at most 65536 words
+    // what a run clears before its scans, as spans of at most 8192 words
     std::vector<ZeroSpan> zs;
     auto zero = [&](uint32_t *p, uint64_t words) {
-        for (uint64_t o = 0; o < words; o += 65536u)
-            zs.push_back(ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(65536u, words - o)});
+        for (uint64_t o = 0; o < words; o += 8192u)
+            zs.push_back(ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(8192u, words - o)});
     };
     zero((uint32_t *)b->ctl.p, 2u);        // next chunk, class 0
     zero((uint32_t *)(b->ctl.p + 4), 2u);  // next chunk, class 1

@@ -217,8 +217,8 @@ int build_zero_spans(ffbf_ctx *c)
         return FFBF_OK;
     std::vector<ffbf::ZeroSpan> zs;
     auto zero = [&](uint32_t *p, uint64_t words) {
-        for (uint64_t o = 0; o < words; o += 65536u)
-            zs.push_back(ffbf::ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(65536u, words - o)});
+        for (uint64_t o = 0; o < words; o += 8192u)
+            zs.push_back(ffbf::ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(8192u, words - o)});
     };
     zero(c->R.p, c->f_words);
     zero(c->cstage.counts.p, c->cstage.n_regions);

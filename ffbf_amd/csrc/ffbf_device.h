@@ -157,7 +157,8 @@ struct PassArgs {
     uint32_t table_mask;
 };
 
-// A run of u32 words to clear (at most 65536).
+// A run of u32 words to clear (a few thousand words each, so that many
+// workgroups share a large buffer).
 struct ZeroSpan {
     uint32_t *p;
     uint32_t words;

@@ -166,12 +166,20 @@ __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filt
     }
 }
 
-// Zeroes a list of spans of u32 words, each at most 65536 words.
+// Zeroes a list of spans of u32 words, one span per workgroup at a time:
+// 16-byte stores from the first 16-byte boundary, single words around it.
 __global__ __launch_bounds__(256) void k_zero_spans(const ZeroSpan *spans, uint32_t n)
 {
     for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
         const ZeroSpan z = spans[c];
-        for (uint32_t i = threadIdx.x; i < z.words; i += blockDim.x) z.p[i] = 0u;
+        const uint32_t head = (uint32_t)(((16u - ((uintptr_t)z.p & 15u)) & 15u) / 4u);
+        const uint32_t h = head < z.words ? head : z.words;
+        const uint32_t n4 = (z.words - h) / 4u;
+        if (threadIdx.x < h) z.p[threadIdx.x] = 0u;
+        uint4 *v = (uint4 *)(z.p + h);
+        for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) v[i] = make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t tail0 = h + n4 * 4u;
+        if (threadIdx.x < z.words - tail0) z.p[tail0 + threadIdx.x] = 0u;
     }
 }
 
@@ -287,10 +295,20 @@ __global__ __launch_bounds__(1024) void k_cand_offsets(uint32_t *counts, uint64_
 {
     __shared__ uint64_t s_part[1024];
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (n + 1023u) / 1024u;
+    // 4-count groups per thread, so that counts move as 16-byte vectors
+    const uint32_t per = ((n + 1023u) / 1024u + 3u) & ~3u;
     const uint32_t lo = t * per, hi = lo + per < n ? lo + per : n;
+    const bool vec = (n & 3u) == 0u;  // then lo, hi are multiples of 4 (counts is 256-byte aligned)
     uint64_t sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += counts[i];
+    if (vec) {
+#pragma unroll 4
+        for (uint32_t i = lo; i < hi; i += 4) {
+            const uint4 v = *(const uint4 *)(counts + i);
+            sum += (uint64_t)v.x + v.y + v.z + v.w;
+        }
+    } else {
+        for (uint32_t i = lo; i < hi; i++) sum += counts[i];
+    }
     s_part[t] = sum;
     __syncthreads();
     for (uint32_t d = 1; d < 1024u; d <<= 1) {
@@ -301,10 +319,23 @@ __global__ __launch_bounds__(1024) void k_cand_offsets(uint32_t *counts, uint64_
     }
     const uint64_t base = counters[C_CAND];
     uint64_t run = base + (t ? s_part[t - 1] : 0u);
-    for (uint32_t i = lo; i < hi; i++) {
-        offsets[i] = run;
-        run += counts[i];
-        counts[i] = 0u;
+    if (vec) {
+#pragma unroll 4
+        for (uint32_t i = lo; i < hi; i += 4) {
+            const uint4 v = *(const uint4 *)(counts + i);
+            offsets[i] = run;
+            offsets[i + 1] = run + v.x;
+            offsets[i + 2] = run + v.x + v.y;
+            offsets[i + 3] = run + v.x + v.y + v.z;
+            run += (uint64_t)v.x + v.y + v.z + v.w;
+            *(uint4 *)(counts + i) = make_uint4(0u, 0u, 0u, 0u);
+        }
+    } else {
+        for (uint32_t i = lo; i < hi; i++) {
+            offsets[i] = run;
+            run += counts[i];
+            counts[i] = 0u;
+        }
     }
     __syncthreads();
     if (t == 0) {
