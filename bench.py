@@ -483,7 +483,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS) + ["batch3", "batch4", "serve5", "ingest"])
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg of the default run")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched, serving and ingest legs of the default run")
     ap.add_argument("--mode", type=int, default=-1, help="scan mode override (-1 auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -621,8 +622,12 @@ def main() -> int:
     if args.workload == "plain" and not args.no_batched:
         log("batched leg (batch3) ...")
         out["batched"] = run_batched(fa, "batch3", steps=10)
+        log("multi-filter batched leg (batch4) ...")
+        out["batched_multi_filter"] = run_batched(fa, "batch4", steps=5, per_text_reps=1)
         log("serving leg (serve5) ...")
         out["serving"] = run_serving(fa, n_requests=600)
+        log("ingest leg (host memory / file) ...")
+        out["ingest"] = run_ingest(fa, reps=2)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, sbox, pb, po, text)

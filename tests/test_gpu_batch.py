@@ -252,3 +252,49 @@ def test_batch_matches_at_unaligned_text_starts():
     dev.free()
     for f in filters:
         f[0].close()
+
+
+def test_batch_run_time_window_length():
+    # a window length without a compiled fast path (the kernels take W at run
+    # time): batch equals the oracle text by text
+    say("batch with W=40")
+    w, k = 40, 3
+    sbox = fa.sbox_from_seed(83)
+    filters = []
+    for i, (n, log2, alpha, mode) in enumerate([(400, 14, fa.ALPHA_DNA_GC40, fa.MODE_LDS),
+                                                (4000, 22, fa.ALPHA_ASCII, fa.MODE_FOLD),
+                                                (30000, 24, fa.ALPHA_DNA_GC40, fa.MODE_GFOLD)]):
+        b, off = fa.gen_patterns(alpha, 850 + i, n, w, 90)
+        m = fa.Matcher(w, k, log2, sbox)
+        m.set_patterns(b, off)
+        m.build()
+        m.set_scan_mode(mode)
+        filters.append((m, b, off, log2))
+    rng = np.random.default_rng(13)
+    texts, route = [], []
+    for i in range(24):
+        f = i % 3
+        n = int(rng.integers(0, 150_000))
+        t = fa.gen_bytes(fa.ALPHA_DNA_GC40 if f != 1 else fa.ALPHA_ASCII, 870 + i, n)
+        if n > 200:
+            fa.plant(880 + i, 1 + n // 20000, t, filters[f][1], filters[f][2])
+        texts.append(t)
+        route.append(f)
+    route = np.array(route, dtype=np.uint32)
+    off = np.zeros(len(texts) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([t.size for t in texts])
+    rows = []
+    for i, t in enumerate(texts):
+        m, b, po, log2 = filters[int(route[i])]
+        r = Oracle(w, k, log2, sbox).run(t, b, po)
+        rows += [(i, int(p), int(q)) for p, q in zip(r["ids"], r["offs"])]
+    expect = sorted(rows)
+    assert len(expect) > 10
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+    dev.free()
+    for f in filters:
+        f[0].close()
