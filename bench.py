@@ -14,7 +14,9 @@ The CPU baseline times the oracle (a C restatement of the reference
 algorithm) on this machine's host cores over a bounded sample of the same
 inputs.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|cfg1|cfg3text]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                      [--workload plain|ascii10k-64m|ascii100k-1536m|batch-4096texts-1set|
+                                  batch-2048texts-64sets|serve-8sets|ingest]
 """
 from __future__ import annotations
 
@@ -33,8 +35,8 @@ METRIC = "text GB/s (+% HBM peak) at 1M patterns, plain vs batched; p99 req late
 WORKLOADS = {
     # name: alphabet, W, k, log2 bits, patterns, lmin, lmax, text bytes, plants, seed
     "plain": ("dna", 32, 4, 28, 1_000_000, 32, 100, 2 << 30, 50_000, 2),
-    "cfg1": ("ascii", 19, 4, 24, 10_000, 20, 40, 64 << 20, 1_000, 1),
-    "cfg3text": ("ascii", 19, 4, 26, 100_000, 20, 64, 1536 << 20, 1_536, 3),
+    "ascii10k-64m": ("ascii", 19, 4, 24, 10_000, 20, 40, 64 << 20, 1_000, 1),
+    "ascii100k-1536m": ("ascii", 19, 4, 26, 100_000, 20, 64, 1536 << 20, 1_536, 3),
 }
 
 
@@ -77,11 +79,11 @@ def next_pow2(x: int) -> int:
 def batch_inputs(fa, name, scale=1.0):
     """Synthetic batch of texts routed to resident pattern sets.
 
-    batch3: 4,096 texts, log-normal sizes (median 256 KiB, sigma 0.9, clipped
+    batch-4096texts-1set: 4,096 texts, log-normal sizes (median 256 KiB, sigma 0.9, clipped
     to 4 KiB-64 MiB, ~1.5 GiB), printable ASCII, one 100,000-pattern set
     (lengths 20-64), W=19, k=4, 2^26-bit filter; one planted occurrence per
     started MiB of text.
-    batch4: 64 pattern sets with n log-uniform in [1e3, 1e6] (lengths 24-80,
+    batch-2048texts-64sets: 64 pattern sets with n log-uniform in [1e3, 1e6] (lengths 24-80,
     DNA), 2,048 DNA texts (log-normal, median 1 MiB, sigma 1.0, clipped to
     4 KiB-64 MiB) with one tandem-repeat run (period 1-6, 205-614 bases, ~10%
     of bases) per 4 KiB, each text routed to a seeded set; W=24, k=3,
@@ -89,7 +91,7 @@ def batch_inputs(fa, name, scale=1.0):
     """
     import numpy as np
 
-    if name == "batch3":
+    if name == "batch-4096texts-1set":
         W, k, seed = 19, 4, 31
         n_texts = max(8, int(4096 * scale))
         sizes = fa.lognormal_sizes(seed, n_texts, 256 * 1024.0, 0.9, 4096, 64 << 20)
@@ -97,7 +99,7 @@ def batch_inputs(fa, name, scale=1.0):
         route = np.zeros(n_texts, dtype=np.uint32)
         alpha_of = lambda f: fa.ALPHA_ASCII
         repeats = False
-    elif name == "batch4":
+    elif name == "batch-2048texts-64sets":
         W, k, seed = 24, 3, 41
         n_texts = max(8, int(2048 * scale))
         sizes = fa.lognormal_sizes(seed, n_texts, 1024 * 1024.0, 1.0, 4096, 64 << 20)
@@ -324,7 +326,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=2):
     d["requests"] = None
     d["pinned"].free()
     return {
-        "workload": "serve5",
+        "workload": "serve-8sets",
         "requests": n_requests,
         "text_bytes": int(total),
         "filters": 8,
@@ -482,7 +484,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=sorted(WORKLOADS) + ["batch3", "batch4", "serve5", "ingest"])
+    ap.add_argument("--workload", default="plain",
+                    choices=sorted(WORKLOADS) + ["batch-4096texts-1set", "batch-2048texts-64sets", "serve-8sets", "ingest"])
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched, serving and ingest legs of the default run")
     ap.add_argument("--mode", type=int, default=-1, help="scan mode override (-1 auto)")
@@ -528,16 +531,16 @@ def main() -> int:
                           "config": {"workload": "ingest", "text_bytes": r["text_bytes"]}, "ingest": r}), flush=True)
         return 0
 
-    if args.workload == "serve5":
+    if args.workload == "serve-8sets":
         r = run_serving(fa, n_requests=max(64, args.steps * 100))
         print(json.dumps({"metric": METRIC, "value": r["saturation_gbs"], "unit": "GB/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "dtype": "u32",
                           "data": "synthetic (seeded counter-based generator)",
-                          "config": {"workload": "serve5", "requests": r["requests"], "text_bytes": r["text_bytes"]},
+                          "config": {"workload": "serve-8sets", "requests": r["requests"], "text_bytes": r["text_bytes"]},
                           "serving": r}), flush=True)
         return 0
-    if args.workload in ("batch3", "batch4"):
+    if args.workload in ("batch-4096texts-1set", "batch-2048texts-64sets"):
         r = run_batched(fa, args.workload, steps=args.steps)
         print(json.dumps({"metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
                           "warmup": args.warmup, "ms_per_step": r["ms_per_batch"], "higher_is_better": True,
@@ -620,11 +623,11 @@ def main() -> int:
     m.close()
     dev.free()
     if args.workload == "plain" and not args.no_batched:
-        log("batched leg (batch3) ...")
-        out["batched"] = run_batched(fa, "batch3", steps=10)
-        log("multi-filter batched leg (batch4) ...")
-        out["batched_multi_filter"] = run_batched(fa, "batch4", steps=5, per_text_reps=1)
-        log("serving leg (serve5) ...")
+        log("batched leg (batch-4096texts-1set) ...")
+        out["batched"] = run_batched(fa, "batch-4096texts-1set", steps=10)
+        log("multi-filter batched leg (batch-2048texts-64sets) ...")
+        out["batched_multi_filter"] = run_batched(fa, "batch-2048texts-64sets", steps=5, per_text_reps=1)
+        log("serving leg (serve-8sets) ...")
         out["serving"] = run_serving(fa, n_requests=600)
         log("ingest leg (host memory / file) ...")
         out["ingest"] = run_ingest(fa, reps=2)
