@@ -228,10 +228,10 @@ int enqueue_run(ffbf_batch *b, bool timed)
         ba.ctl = b->ctl.p + 4;
         HIP_TRY(launch_scan_batch_global(ba, b->n_cu, b->stream));
     }
+    if (timed) HIP_TRY(hipEventRecord(b->e_scan1, b->stream));  // the scan kernels alone end here
     HIP_TRY(launch_cand_compact(b->cstage.raw.p, b->cstage.raw.cap, b->cstage.counts.p, b->cstage.offsets.p,
                                 b->cstage.n_regions, b->cstage.region, b->cand.p, b->cand.cap, b->counters.p, b->n_cu,
                                 b->stream));
-    if (timed) HIP_TRY(hipEventRecord(b->e_scan1, b->stream));
     HIP_TRY(launch_pattern_pass_multi(b->d_pass.p, b->d_wave_start.p, (uint32_t)b->filters.size(), b->d_active.p,
                                       b->pass_waves, b->n_cu, b->stream));
     BatchVerifyArgs va{};

@@ -57,7 +57,9 @@ struct DevBuf {
     }
 };
 
-enum Ev { EV_B0, EV_B1, EV_S0, EV_S1, EV_S2, EV_P0, EV_P1, EV_V0, EV_V1, EV_N };
+// Stage events: build B0-B1; scan S0 (clears) S1 (kernel) S2 (compaction) S3;
+// pattern pass P0-P1; verification V0-V1.
+enum Ev { EV_B0, EV_B1, EV_S0, EV_S1, EV_S2, EV_S3, EV_P0, EV_P1, EV_V0, EV_V1, EV_N };
 
 // Where a scan launch stages its candidates before they are compacted into
 // the candidate list: one region per wave slot plus a spill area (ScanArgs),
@@ -186,7 +188,7 @@ int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n);
 int record(ffbf_ctx *c, int e);
 int sync_counters(ffbf_ctx *c);
 int enqueue_scan_begin(ffbf_ctx *c);
-int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base);
+int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end);
 int do_scan(ffbf_ctx *c);
 // Grows the candidate list (and its staging) to hold at least n entries.
 int ensure_cand(ffbf_ctx *c, uint64_t n);

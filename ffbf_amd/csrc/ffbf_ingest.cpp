@@ -208,7 +208,7 @@ int ingest(ffbf_ctx *c, Source &src, uint64_t size_hint)
             // every window starting before copied-W+1 has all its bytes here
             const uint64_t end = copied - W + 1;
             HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_staged[s], 0));
-            RET_IF(enqueue_scan_range(c, c->owned_text.p + scanned, end - scanned + W - 1, scanned));
+            RET_IF(enqueue_scan_range(c, c->owned_text.p + scanned, end - scanned + W - 1, scanned, false));
             scanned = end;
         }
     }
@@ -220,6 +220,7 @@ int ingest(ffbf_ctx *c, Source &src, uint64_t size_hint)
         return st;
     }
     RET_IF(record(c, EV_S2));
+    RET_IF(record(c, EV_S3));
     c->text = c->owned_text.p;
     c->text_n = copied;
     RET_IF(sync_counters(c));

@@ -290,17 +290,13 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
                                                       uint64_t tag, uint4 *q, uint32_t &qn, PendingCand &pend)
 {
     using QG = QueueGeom<BLOCK>;
-    uint32_t vm = 0xFFFFu;
-    if (lane_id() >= (int)kIterUnits) {
-        vm = 0;  // these units belong to the next wave-iteration
-    } else if (p0 < 0 || (uint64_t)p0 + kUnit > npos) {
-        vm = 0;
-#pragma unroll
-        for (int b = 0; b < kUnit; b++) {
-            const int64_t p = p0 + b;
-            vm |= (uint32_t)(p >= 0 && (uint64_t)p < npos) << b;
-        }
-    }
+    // the real windows of this lane (0 <= p0 + b < npos) are one run [lo, hi)
+    const int64_t lo64 = p0 < 0 ? -p0 : 0;
+    const int64_t hi64 = (int64_t)npos - p0;
+    const uint32_t lo = (uint32_t)(lo64 < kUnit ? lo64 : kUnit);
+    const uint32_t hi = (uint32_t)(hi64 <= 0 ? 0 : (hi64 < kUnit ? hi64 : kUnit));
+    uint32_t vm = hi > lo ? ((0xFFFFu >> (kUnit - hi)) & (0xFFFFu << lo)) : 0u;
+    if (lane_id() >= (int)kIterUnits) vm = 0;  // these units belong to the next wave-iteration
     uint32_t pm = 0;
     if constexpr (LDSPRE) {
         // P bits per window, all loads in flight together

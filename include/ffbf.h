@@ -73,7 +73,7 @@ typedef struct ffbf_match {
  * measured with HIP events on the context's stream. */
 typedef struct ffbf_timings {
     float build_ms;        /* filter build (clear F, hash prefixes, set bits) */
-    float scan_ms;         /* clear R + scan with feed-forward */
+    float scan_ms;         /* clears + scan with feed-forward + candidate compaction */
     float scan_kernel_ms;  /* the scan kernel alone */
     float pattern_pass_ms; /* pattern pass (+ survivor index) */
     float verify_ms;       /* exact verification */
