@@ -38,6 +38,7 @@ struct Request {
     std::string error;
     std::vector<ffbf_match> matches;
     bool done = false;
+    bool claimed = false;  // a wait holds it (one waiter per request)
 };
 
 struct Slot {
@@ -314,8 +315,10 @@ int ffbf_server_wait(ffbf_server *s, uint64_t request_id, ffbf_match **matches, 
         auto it = s->reqs.find(request_id);
         if (it == s->reqs.end()) return fail(FFBF_EINVAL, "unknown request");
         r = it->second;
+        if (r->claimed) return fail(FFBF_EINVAL, "request already being waited for");
+        r->claimed = true;
         s->cv_done.wait(lk, [&] { return r->done; });
-        s->reqs.erase(it);
+        s->reqs.erase(request_id);
     }
     const int st = r->status;
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();

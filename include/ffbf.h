@@ -297,7 +297,8 @@ int ffbf_server_submit_many(ffbf_server *server, uint32_t n, const uint32_t *fil
 int ffbf_server_poll(ffbf_server *server, uint64_t request_id, int *done);
 /* Blocks until the request completes; *matches (sorted by pattern id, then
  * offset) is allocated by the library (ffbf_free_matches); latency_ms is
- * submit-to-completion.  The request id is released. */
+ * submit-to-completion.  The request id is released.  One wait per request:
+ * a second concurrent wait on the same id returns FFBF_EINVAL. */
 int ffbf_server_wait(ffbf_server *server, uint64_t request_id, ffbf_match **matches, uint64_t *n, double *latency_ms);
 int ffbf_server_get_stats(ffbf_server *server, ffbf_server_stats *out);
 

@@ -143,5 +143,27 @@ def test_server_admission_limits():
             got, _ = srv.wait(rid)
             assert np.array_equal(got["offset"], expect[i]["offset"]), i
         assert srv.stats()["rejected"] == 1
+        # two waits on one request: exactly one gets the matches, the other
+        # an error (already waited for, or already released)
+        f0, t0 = reqs[0]
+        rid = srv.submit(f0, t0)
+        res = []
+
+        def waiter():
+            try:
+                res.append(srv.wait(rid)[0])
+            except fa.FFBFError as err:
+                res.append(err.status)
+
+        ths = [threading.Thread(target=waiter) for _ in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        errs = [x for x in res if isinstance(x, int)]
+        oks = [x for x in res if not isinstance(x, int)]
+        assert errs == [fa.EINVAL] and len(oks) == 1
+        got = oks[0]
+        assert np.array_equal(got["offset"], expect[0]["offset"])
     for f in filters:
         f[0].close()
