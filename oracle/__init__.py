@@ -21,6 +21,18 @@ reference, which does not build here):
   candidate_lines / survivor_files   hand-built texts and pattern sets
                     (newline runs, a line without '\n', a candidate on a line
                     break, several candidates in one line)
+  split method (SplitOracle, the CPU binary's filter)
+    hash            closed form XOR of 64-bit rotations (golden vectors from
+                    the reference's 64-bit sbox table), the W=64 all-rotations
+                    parity identity, W=1 = the sbox entry, 0x80..0xFF hashing
+                    as 0x00
+    indices         hand-evaluated values at (h1,h2) = (1,0), (0,1) and
+                    all-ones, cache/external placement by s
+    build/scan/pass brute-force Python set semantics (windows without a
+                    newline) on small texts with newlines and high bytes;
+                    threaded scan equals the single-threaded one
+    verify          naive matcher restricted to patterns whose prefix window
+                    has no newline
 """
 from __future__ import annotations
 
@@ -68,6 +80,16 @@ _sig = {
     "orc_brute_matches": (_u64, [_P, _u64, _P, _P, _u64, _u32, _P, _P, _u64]),
     "orc_filter_save_deltas": (_u64, [_P, _u32, _P, _u64]),
     "orc_filter_load_deltas": (None, [_P, _u64, _u32, _P]),
+    "orc_filter_save_deltas_bits": (_u64, [_P, _u64, _P, _u64]),
+    "orc_split_bits": (_u64, [_P]),
+    "orc_split_hash_window": (None, [_P, _P, _P, _P]),
+    "orc_split_fwd_indices": (None, [_P, _u32, _u32, _P]),
+    "orc_split_rev_indices": (None, [_P, _u32, _u32, _P]),
+    "orc_split_build": (_u64, [_P, _P, _P, _u64, _P]),
+    "orc_split_scan": (_u64, [_P, _P, _u64, _P, _P, _P, _u64]),
+    "orc_split_scan_mt": (_u64, [_P, _P, _u64, _P, _P, _P, _u64, ctypes.c_int]),
+    "orc_split_pattern_pass": (_u64, [_P, _P, _P, _u64, _P, _P]),
+    "orc_split_verify": (_u64, [_P, _P, _u64, _P, _P, _u64, _P, _u64, _P, _P, _P, _u64]),
 }
 for _n, (_r, _a) in _sig.items():
     _f = getattr(_lib, _n)
@@ -161,6 +183,102 @@ class Oracle:
         surv = self.pattern_pass(pat_bytes, pat_off, R)
         ids, offs = self.verify(text, pat_bytes, pat_off, cand, surv)
         return {"F": F, "R": R, "cand": cand, "surv": surv, "ids": ids, "offs": offs}
+
+
+class SplitParams(ctypes.Structure):
+    _fields_ = [
+        ("window", ctypes.c_uint32),
+        ("k", ctypes.c_uint32),
+        ("s", ctypes.c_uint32),
+        ("log2_cache_bits", ctypes.c_uint32),
+        ("log2_ext_bits", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("sbox", ctypes.c_uint64 * 256),
+    ]
+
+
+class SplitOracle(Oracle):
+    """The CPU binary's split filter (src/rabin-karp/ffbf.cpp): 64-bit sbox
+    hash, k functions of which the first s address a 2^c-bit cache part and
+    the rest a 2^e-bit external part, hashing restarted at every newline and
+    bytes >= 0x80 hashed as 0.  Same stage interface as Oracle."""
+
+    def __init__(self, window: int, k: int, s: int, log2_cache_bits: int, log2_ext_bits: int, sbox):
+        self.p = SplitParams()
+        self.p.window, self.p.k, self.p.s = window, k, s
+        self.p.log2_cache_bits, self.p.log2_ext_bits, self.p.reserved = log2_cache_bits, log2_ext_bits, 0
+        sb = np.asarray(sbox, dtype=np.uint64)
+        for i in range(256):
+            self.p.sbox[i] = int(sb[i])
+        self.window, self.k, self.s = window, k, s
+        self.log2_cache_bits, self.log2_ext_bits = log2_cache_bits, log2_ext_bits
+        self.n_bits = (1 << log2_cache_bits) + (1 << log2_ext_bits)
+        self.n_words = self.n_bits // 32
+        self._pp = ctypes.byref(self.p)
+
+    def hash_window(self, win: bytes):
+        w = np.frombuffer(bytes(win), dtype=np.uint8)
+        h1, h2 = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        _lib.orc_split_hash_window(self._pp, _ptr(w), ctypes.byref(h1), ctypes.byref(h2))
+        return h1.value, h2.value
+
+    def hash_rolling(self, text):
+        raise NotImplementedError("the split method hashes line by line; use scan")
+
+    def fwd_indices(self, h1: int, h2: int) -> list:
+        out = np.zeros(11, dtype=np.uint32)
+        _lib.orc_split_fwd_indices(self._pp, h1, h2, _ptr(out))
+        return out[: self.k].tolist()
+
+    def rev_indices(self, h1: int, h2: int) -> list:
+        out = np.zeros(11, dtype=np.uint32)
+        _lib.orc_split_rev_indices(self._pp, h1, h2, _ptr(out))
+        return out[: self.k].tolist()
+
+    def build(self, pat_bytes: np.ndarray, pat_off: np.ndarray) -> np.ndarray:
+        F = np.zeros(self.n_words, dtype=np.uint32)
+        _lib.orc_split_build(self._pp, _ptr(pat_bytes), _ptr(pat_off), pat_off.size - 1, _ptr(F))
+        return F
+
+    def scan(self, text: np.ndarray, F: np.ndarray, threads: int = 1):
+        R = np.zeros(self.n_words, dtype=np.uint32)
+        cap = max(1, min(text.size, 1 << 22))
+        while True:
+            cand = np.zeros(cap, dtype=np.uint64)
+            if threads > 1:
+                n = _lib.orc_split_scan_mt(self._pp, _ptr(text), text.size, _ptr(F), _ptr(R), _ptr(cand), cap,
+                                           threads)
+            else:
+                n = _lib.orc_split_scan(self._pp, _ptr(text), text.size, _ptr(F), _ptr(R), _ptr(cand), cap)
+            if n <= cap:
+                return R, cand[:n]
+            cap = int(n)
+
+    def pattern_pass(self, pat_bytes: np.ndarray, pat_off: np.ndarray, R: np.ndarray) -> np.ndarray:
+        n = pat_off.size - 1
+        bm = np.zeros(max(1, (n + 31) // 32), dtype=np.uint32)
+        _lib.orc_split_pattern_pass(self._pp, _ptr(pat_bytes), _ptr(pat_off), n, _ptr(R), _ptr(bm))
+        return bm[: (n + 31) // 32]
+
+    def verify(self, text, pat_bytes, pat_off, cand, surv_bitmap):
+        n = pat_off.size - 1
+        sb = surv_bitmap if surv_bitmap.size else np.zeros(1, dtype=np.uint32)
+        cap = 1 << 16
+        while True:
+            ids = np.zeros(cap, dtype=np.uint32)
+            offs = np.zeros(cap, dtype=np.uint64)
+            m = _lib.orc_split_verify(self._pp, _ptr(text), text.size, _ptr(pat_bytes), _ptr(pat_off), n,
+                                      _ptr(cand), cand.size, _ptr(sb), _ptr(ids), _ptr(offs), cap)
+            if m <= cap:
+                return ids[:m], offs[:m]
+            cap = int(m)
+
+
+def filter_save_deltas_bits(F: np.ndarray, n_bits: int) -> np.ndarray:
+    n = _lib.orc_filter_save_deltas_bits(_ptr(F), n_bits, None, 0)
+    out = np.zeros(max(int(n), 1), dtype=np.uint32)
+    _lib.orc_filter_save_deltas_bits(_ptr(F), n_bits, _ptr(out), n)
+    return out[: int(n)]
 
 
 def brute_matches(text: np.ndarray, pat_bytes: np.ndarray, pat_off: np.ndarray, min_len: int):

@@ -274,16 +274,17 @@ static int cmp_match(const void *x, const void *y)
     return a->off < b->off ? -1 : (a->off > b->off);
 }
 
+typedef void (*window_hash_fn)(const void *p, const uint8_t *win, uint32_t *h1, uint32_t *h2);
+
 /* Exact verification.  The reference hands survivors and filtered lines to
  * `grep -Ff` (src/rabin-karp/README:49-52); this restates what that step
  * decides at (pattern id, offset) granularity: join candidates to survivors
  * on the prefix-window hash pair, then compare every byte. */
-uint64_t orc_verify(const orc_params *p, const uint8_t *text, uint64_t n,
-                    const uint8_t *bytes, const uint64_t *off, uint64_t n_patterns,
-                    const uint64_t *cand, uint64_t n_cand, const uint32_t *surv_bitmap,
-                    uint32_t *out_id, uint64_t *out_off, uint64_t cap)
+static uint64_t verify_core(const void *p, window_hash_fn hash, uint32_t W, const uint8_t *text, uint64_t n,
+                            const uint8_t *bytes, const uint64_t *off, uint64_t n_patterns,
+                            const uint64_t *cand, uint64_t n_cand, const uint32_t *surv_bitmap,
+                            uint32_t *out_id, uint64_t *out_off, uint64_t cap)
 {
-    const uint32_t W = p->window;
     uint64_t ns = 0;
     for (uint64_t i = 0; i < n_patterns; i++)
         if ((surv_bitmap[i >> 5] >> (i & 31u)) & 1u) ns++;
@@ -292,7 +293,7 @@ uint64_t orc_verify(const orc_params *p, const uint8_t *text, uint64_t n,
     for (uint64_t i = 0; i < n_patterns; i++) {
         if (!((surv_bitmap[i >> 5] >> (i & 31u)) & 1u)) continue;
         uint32_t h1, h2;
-        orc_hash_window(p, bytes + off[i], &h1, &h2);
+        hash(p, bytes + off[i], &h1, &h2);
         tab[t].key = ((uint64_t)h1 << 32) | h2;
         tab[t].id = (uint32_t)i;
         tab[t].pad = 0;
@@ -306,7 +307,7 @@ uint64_t orc_verify(const orc_params *p, const uint8_t *text, uint64_t n,
         uint64_t o = cand[c];
         if (o + W > n) continue;
         uint32_t h1, h2;
-        orc_hash_window(p, text + o, &h1, &h2);
+        hash(p, text + o, &h1, &h2);
         uint64_t key = ((uint64_t)h1 << 32) | h2;
         /* lower bound */
         uint64_t lo = 0, hi = ns;
@@ -337,6 +338,20 @@ uint64_t orc_verify(const orc_params *p, const uint8_t *text, uint64_t n,
     free(m);
     free(tab);
     return nm;
+}
+
+static void pair_hash(const void *p, const uint8_t *win, uint32_t *h1, uint32_t *h2)
+{
+    orc_hash_window((const orc_params *)p, win, h1, h2);
+}
+
+uint64_t orc_verify(const orc_params *p, const uint8_t *text, uint64_t n,
+                    const uint8_t *bytes, const uint64_t *off, uint64_t n_patterns,
+                    const uint64_t *cand, uint64_t n_cand, const uint32_t *surv_bitmap,
+                    uint32_t *out_id, uint64_t *out_off, uint64_t cap)
+{
+    return verify_core(p, pair_hash, p->window, text, n, bytes, off, n_patterns, cand, n_cand, surv_bitmap,
+                       out_id, out_off, cap);
 }
 
 uint64_t orc_brute_matches(const uint8_t *text, uint64_t n,
@@ -388,4 +403,278 @@ void orc_filter_load_deltas(const uint32_t *deltas, uint64_t n, uint32_t log2_bi
         prev += deltas[i];
         F[prev >> 5] |= 1u << (prev & 31u);
     }
+}
+
+uint64_t orc_filter_save_deltas_bits(const uint32_t *F, uint64_t n_bits, uint32_t *out, uint64_t cap)
+{
+    uint32_t prev = 0;
+    uint64_t nout = 0;
+    for (uint64_t i = 0; i < n_bits; i++) {
+        if ((F[i >> 5] >> (i & 31u)) & 1u) {
+            if (nout < cap) out[nout] = (uint32_t)i - prev;
+            prev = (uint32_t)i;
+            nout++;
+        }
+    }
+    return nout;
+}
+
+/* ---------------- split method (src/rabin-karp/ffbf.cpp) ---------------- */
+
+static inline uint64_t rotl64(uint64_t x, uint32_t r)
+{
+    r &= 63u;
+    return r ? (x << r) | (x >> (64u - r)) : x;
+}
+
+uint64_t orc_split_bits(const orc_split_params *p)
+{
+    return (1ull << p->log2_cache_bits) + (1ull << p->log2_ext_bits);
+}
+
+/* The rolling hash state of hash_rot_sbox_pre_2 (hash_rot_sbox_pre_2.h:8-51):
+ * h, a ring of the last W rotated sbox values and the count since reset. */
+typedef struct split_hash {
+    uint64_t h;
+    uint64_t ring[64];
+    uint32_t ptr;
+    int full;
+} split_hash;
+
+/* reset (hash_rot_sbox_pre_2.h:29-36) */
+static inline void split_reset(split_hash *st)
+{
+    st->h = 0;
+    st->ptr = 0;
+    st->full = 0;
+    memset(st->ring, 0, sizeof st->ring);
+}
+
+/* updateHashes (ffbf.cpp:124-133): bytes >= 0x80 become 0, then
+ * hash_rot_sbox_pre_2::update (:41-50): rotate by one, add the new byte's
+ * sbox value, remove the value W bytes back (kept pre-rotated by W). */
+static inline void split_update(const orc_split_params *p, split_hash *st, uint8_t c)
+{
+    if (c & 0x80u) c = 0;
+    const uint64_t sc = p->sbox[c];
+    st->h = rotl64(st->h, 1) ^ sc ^ st->ring[st->ptr];
+    st->ring[st->ptr] = rotl64(sc, p->window);
+    if (++st->ptr >= p->window) {
+        st->ptr = 0;
+        st->full = 1;
+    }
+}
+
+void orc_split_hash_window(const orc_split_params *p, const uint8_t *win, uint32_t *h1, uint32_t *h2)
+{
+    split_hash st;
+    split_reset(&st);
+    for (uint32_t j = 0; j < p->window; j++) split_update(p, &st, win[j]);
+    *h1 = (uint32_t)st.h;          /* hval1 (:26) */
+    *h2 = (uint32_t)(st.h >> 32);  /* hval2 (:27) */
+}
+
+static inline uint32_t rot16(uint32_t x) { return (x << 16) | (x >> 16); }
+
+static inline uint32_t split_place(const orc_split_params *p, uint32_t i, uint32_t x)
+{
+    if (i < p->s) return x & (uint32_t)((1ull << p->log2_cache_bits) - 1u);
+    return (x & (uint32_t)((1ull << p->log2_ext_bits) - 1u)) + (uint32_t)(1ull << p->log2_cache_bits);
+}
+
+/* getHashValue (ffbf.cpp:135-159) */
+void orc_split_fwd_indices(const orc_split_params *p, uint32_t h1, uint32_t h2, uint32_t *out)
+{
+    uint32_t f[11];
+    f[0] = h1;
+    f[1] = h2;
+    f[2] = rot16(h1);
+    f[3] = rot16(h2);
+    f[4] = h1 + h2;
+    f[5] = h1 + (h2 << 1);
+    f[6] = h1 + (h2 << 2);
+    f[7] = h1 + 3u * h2;
+    f[8] = h1 + 5u * h2;
+    f[9] = h1 + 6u * h2;
+    f[10] = h1 + 7u * h2;
+    for (uint32_t i = 0; i < p->k; i++) out[i] = split_place(p, i, f[i]);
+}
+
+/* getHashValue_2 (ffbf.cpp:161-185) */
+void orc_split_rev_indices(const orc_split_params *p, uint32_t h1, uint32_t h2, uint32_t *out)
+{
+    uint32_t r[11];
+    r[0] = h1 + (h2 << 1);
+    r[1] = h1 + (h2 << 2);
+    r[2] = h1 + 3u * h2;
+    r[3] = h1 + 5u * h2;
+    r[4] = h1 + 6u * h2;
+    r[5] = h1 + 7u * h2;
+    r[6] = h1;
+    r[7] = h2;
+    r[8] = rot16(h1);
+    r[9] = rot16(h2);
+    r[10] = h1 + h2;
+    for (uint32_t i = 0; i < p->k; i++) out[i] = split_place(p, i, r[i]);
+}
+
+static inline uint64_t split_words(const orc_split_params *p) { return orc_split_bits(p) / 32u; }
+
+/* Filter::operator<< (ffbf.cpp:268-283) per pattern, over a cleared F. */
+uint64_t orc_split_build(const orc_split_params *p, const uint8_t *bytes, const uint64_t *off,
+                         uint64_t n_patterns, uint32_t *F)
+{
+    memset(F, 0, split_words(p) * sizeof(uint32_t));
+    uint64_t hashed = 0;
+    for (uint64_t i = 0; i < n_patterns; i++) {
+        if (off[i + 1] - off[i] < p->window) continue;
+        uint32_t h1, h2, idx[11];
+        orc_split_hash_window(p, bytes + off[i], &h1, &h2);
+        orc_split_fwd_indices(p, h1, h2, idx);
+        for (uint32_t j = 0; j < p->k; j++) set_bit(F, idx[j]);
+        hashed++;
+    }
+    return hashed;
+}
+
+/* Filter::processFile's loop (ffbf.cpp:340-370) over the window starts
+ * [lo, hi): restart at '\n', hash every other byte, test F once the hash is
+ * full (checkInFilter, :214-239) and on a hit set the reverse bits in R
+ * (setBitsInFilter_2, :205-212).  Starting from a reset at lo is exact: a
+ * window's hash depends only on its own W bytes. */
+static uint64_t split_scan_range(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                                 uint64_t lo, uint64_t hi, const uint32_t *F, uint32_t *R,
+                                 uint64_t *cand, uint64_t cap)
+{
+    const uint32_t W = p->window;
+    uint64_t found = 0;
+    if (n < W || lo >= hi) return 0;
+    const uint64_t end = hi + W - 1 < n ? hi + W - 1 : n;
+    split_hash st;
+    split_reset(&st);
+    for (uint64_t i = lo; i < end; i++) {
+        if (text[i] == '\n') {
+            split_reset(&st);
+            continue;
+        }
+        split_update(p, &st, text[i]);
+        if (!st.full) continue;
+        const uint32_t h1 = (uint32_t)st.h, h2 = (uint32_t)(st.h >> 32);
+        uint32_t idx[11];
+        orc_split_fwd_indices(p, h1, h2, idx);
+        int hit = 1;
+        for (uint32_t j = 0; j < p->k; j++) {
+            if (!test_bit(F, idx[j])) { hit = 0; break; }
+        }
+        if (!hit) continue;
+        orc_split_rev_indices(p, h1, h2, idx);
+        for (uint32_t j = 0; j < p->k; j++) set_bit(R, idx[j]);
+        if (found < cap) cand[found] = i + 1 - W;
+        found++;
+    }
+    return found;
+}
+
+uint64_t orc_split_scan(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                        const uint32_t *F, uint32_t *R, uint64_t *cand, uint64_t cap)
+{
+    memset(R, 0, split_words(p) * sizeof(uint32_t));
+    if (n < p->window) return 0;
+    return split_scan_range(p, text, n, 0, n - p->window + 1, F, R, cand, cap);
+}
+
+typedef struct split_job {
+    const orc_split_params *p;
+    const uint8_t *text;
+    uint64_t n, lo, hi;
+    const uint32_t *F;
+    uint32_t *R;
+    uint64_t *cand;
+    uint64_t cap, found;
+} split_job;
+
+static void *split_worker(void *arg)
+{
+    split_job *j = (split_job *)arg;
+    j->found = split_scan_range(j->p, j->text, j->n, j->lo, j->hi, j->F, j->R, j->cand, j->cap);
+    return NULL;
+}
+
+uint64_t orc_split_scan_mt(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                           const uint32_t *F, uint32_t *R, uint64_t *cand, uint64_t cap, int nthreads)
+{
+    const uint64_t words = split_words(p);
+    memset(R, 0, words * sizeof(uint32_t));
+    if (n < p->window) return 0;
+    const uint64_t npos = n - p->window + 1;
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > npos) nthreads = (int)npos;
+    split_job *jobs = (split_job *)calloc((size_t)nthreads, sizeof(split_job));
+    pthread_t *tid = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    const uint64_t per = (npos + (uint64_t)nthreads - 1) / (uint64_t)nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        split_job *j = &jobs[t];
+        j->p = p; j->text = text; j->n = n; j->F = F;
+        j->lo = (uint64_t)t * per;
+        j->hi = j->lo + per < npos ? j->lo + per : npos;
+        j->R = t == 0 ? R : (uint32_t *)calloc(words, sizeof(uint32_t));
+        j->cap = j->hi > j->lo ? j->hi - j->lo : 0;
+        if (j->cap > cap) j->cap = cap;
+        j->cand = (uint64_t *)malloc((j->cap ? j->cap : 1) * sizeof(uint64_t));
+        pthread_create(&tid[t], NULL, split_worker, j);
+    }
+    uint64_t total = 0;
+    for (int t = 0; t < nthreads; t++) {
+        split_job *j = &jobs[t];
+        pthread_join(tid[t], NULL);
+        if (t > 0) {
+            for (uint64_t w = 0; w < words; w++) R[w] |= j->R[w];
+            free(j->R);
+        }
+        uint64_t kept = j->found < j->cap ? j->found : j->cap;
+        for (uint64_t i = 0; i < kept && total + i < cap; i++) cand[total + i] = j->cand[i];
+        total += j->found;
+        free(j->cand);
+    }
+    free(jobs);
+    free(tid);
+    return total;
+}
+
+/* Filter::filterPhrases (ffbf.cpp:394-485): the first full window of each
+ * pattern tested against R (checkInFilter_2, :241-266); ids are positions. */
+uint64_t orc_split_pattern_pass(const orc_split_params *p, const uint8_t *bytes, const uint64_t *off,
+                                uint64_t n_patterns, const uint32_t *R, uint32_t *surv_bitmap)
+{
+    memset(surv_bitmap, 0, ((n_patterns + 31) / 32) * sizeof(uint32_t));
+    uint64_t survivors = 0;
+    for (uint64_t i = 0; i < n_patterns; i++) {
+        if (off[i + 1] - off[i] < p->window) continue;
+        uint32_t h1, h2, idx[11];
+        orc_split_hash_window(p, bytes + off[i], &h1, &h2);
+        orc_split_rev_indices(p, h1, h2, idx);
+        int ok = 1;
+        for (uint32_t j = 0; j < p->k; j++) {
+            if (!test_bit(R, idx[j])) { ok = 0; break; }
+        }
+        if (ok) {
+            surv_bitmap[i >> 5] |= 1u << (i & 31u);
+            survivors++;
+        }
+    }
+    return survivors;
+}
+
+static void split_hash_cb(const void *p, const uint8_t *win, uint32_t *h1, uint32_t *h2)
+{
+    orc_split_hash_window((const orc_split_params *)p, win, h1, h2);
+}
+
+uint64_t orc_split_verify(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                          const uint8_t *bytes, const uint64_t *off, uint64_t n_patterns,
+                          const uint64_t *cand, uint64_t n_cand, const uint32_t *surv_bitmap,
+                          uint32_t *out_id, uint64_t *out_off, uint64_t cap)
+{
+    return verify_core(p, split_hash_cb, p->window, text, n, bytes, off, n_patterns, cand, n_cand, surv_bitmap,
+                       out_id, out_off, cap);
 }

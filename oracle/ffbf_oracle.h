@@ -92,6 +92,70 @@ uint64_t orc_brute_matches(const uint8_t *text, uint64_t n,
 uint64_t orc_filter_save_deltas(const uint32_t *F, uint32_t log2_bits, uint32_t *out, uint64_t cap);
 void orc_filter_load_deltas(const uint32_t *deltas, uint64_t n, uint32_t log2_bits, uint32_t *F);
 
+/* ---- Split method: the CPU binary's filter (src/rabin-karp/ffbf.cpp) ----
+ *
+ * A 64-bit rolling hash over the window (hash_rot_sbox_pre_2.h:34-51,
+ * 64-bit sbox :55-120): h = rol64(h,1) ^ S[c] ^ rol64(S[c_out], W), whose low
+ * and high halves are h1 and h2.  Bytes >= 0x80 hash as 0 (ffbf.cpp:124-133).
+ * The filter has a cache part of 2^c bits followed by an external part of
+ * 2^e bits; hash function i < s addresses the cache part (x & (2^c-1)), the
+ * others the external part ((x & (2^e-1)) + 2^c) (ffbf.cpp:135-185).  A text
+ * is hashed line by line: the hash restarts after every '\n', so a window
+ * containing '\n' is never tested (ffbf.cpp:340-370).  Rotation amounts are
+ * taken mod 64 (the reference's rol64 shifts by 64 for W = 64, which C leaves
+ * undefined, hash_common.h:13-16). */
+typedef struct orc_split_params {
+    uint32_t window;          /* W (LEN; DEF_PATT_LEN 19), 1..64 */
+    uint32_t k;               /* hash functions per filter, 1..11 (DEF_K 5) */
+    uint32_t s;               /* functions in the cache part, 0..k (DEF_S 2) */
+    uint32_t log2_cache_bits; /* cache part 2^c bits, 5..31 (DEF_CACHE_SIZE 2^24) */
+    uint32_t log2_ext_bits;   /* external part 2^e bits, 5..31 (DEF_EXT_SIZE 2^28) */
+    uint32_t reserved;
+    uint64_t sbox[256];
+} orc_split_params;
+
+/* Bits of a split filter: 2^c + 2^e. */
+uint64_t orc_split_bits(const orc_split_params *p);
+
+/* Hash of one W-byte window by the reference's ring-buffer update from a
+ * reset state (hash_rot_sbox_pre_2.h:32-51, after ffbf.cpp:124-133). */
+void orc_split_hash_window(const orc_split_params *p, const uint8_t *win, uint32_t *h1, uint32_t *h2);
+
+/* Forward (getHashValue, ffbf.cpp:135-159) and reverse (getHashValue_2,
+ * ffbf.cpp:161-185) bit indices, placed in the cache or external part. */
+void orc_split_fwd_indices(const orc_split_params *p, uint32_t h1, uint32_t h2, uint32_t *out);
+void orc_split_rev_indices(const orc_split_params *p, uint32_t h1, uint32_t h2, uint32_t *out);
+
+/* Filter build (Filter::operator<<, ffbf.cpp:268-283): clears F, then sets
+ * k bits for the first W bytes of every pattern with length >= W. */
+uint64_t orc_split_build(const orc_split_params *p, const uint8_t *bytes, const uint64_t *off,
+                         uint64_t n_patterns, uint32_t *F);
+
+/* Scan with feed-forward (Filter::processFile, ffbf.cpp:310-392): clears R;
+ * every window o in [0, n-W] without a '\n' whose k forward bits are set in
+ * F is a candidate, sets its k reverse bits in R and is recorded (ascending,
+ * at most cap).  Returns the number of candidates. */
+uint64_t orc_split_scan(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                        const uint32_t *F, uint32_t *R, uint64_t *cand, uint64_t cap);
+/* The same over nthreads POSIX threads (ranges of window starts). */
+uint64_t orc_split_scan_mt(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                           const uint32_t *F, uint32_t *R, uint64_t *cand, uint64_t cap, int nthreads);
+
+/* Pattern pass (Filter::filterPhrases, ffbf.cpp:394-485): each pattern with
+ * length >= W whose k reverse bits of its prefix window are set in R. */
+uint64_t orc_split_pattern_pass(const orc_split_params *p, const uint8_t *bytes, const uint64_t *off,
+                                uint64_t n_patterns, const uint32_t *R, uint32_t *surv_bitmap);
+
+/* Exact verification as orc_verify, with the split method's window hash. */
+uint64_t orc_split_verify(const orc_split_params *p, const uint8_t *text, uint64_t n,
+                          const uint8_t *bytes, const uint64_t *off, uint64_t n_patterns,
+                          const uint64_t *cand, uint64_t n_cand, const uint32_t *surv_bitmap,
+                          uint32_t *out_id, uint64_t *out_off, uint64_t cap);
+
+/* Filter persistence over a filter of n_bits bits (the split filter's
+ * 2^c + 2^e bits; ffbf.cpp:487-527). */
+uint64_t orc_filter_save_deltas_bits(const uint32_t *F, uint64_t n_bits, uint32_t *out, uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,11 @@ def ref_sbox():
     return np.array(load_golden("ref_sbox.json")["sbox"], dtype=np.uint32)
 
 
+@pytest.fixture(scope="session")
+def ref_sbox64():
+    return np.array([int(x) for x in load_golden("ref_sbox64.json")["sbox"]], dtype=np.uint64)
+
+
 def bits_of(words: np.ndarray) -> list:
     """Sorted indices of set bits of a uint32 word array (bit b in word b>>5)."""
     w = np.asarray(words, dtype=np.uint32)
