@@ -63,6 +63,19 @@ class Params(ctypes.Structure):
     ]
 
 
+class SplitParams(ctypes.Structure):
+    """ffbf_split_params: the CPU binary's split filter (64-bit sbox)."""
+    _fields_ = [
+        ("window", ctypes.c_uint32),
+        ("k", ctypes.c_uint32),
+        ("s", ctypes.c_uint32),
+        ("log2_cache_bits", ctypes.c_uint32),
+        ("log2_ext_bits", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("sbox", ctypes.c_uint64 * 256),
+    ]
+
+
 class Timings(ctypes.Structure):
     _fields_ = [
         ("build_ms", ctypes.c_float),
@@ -106,6 +119,7 @@ ABI = {
     "ffbf_last_error": (_c.c_char_p, []),
     "ffbf_version": (_c.c_char_p, []),
     "ffbf_create": (_i32, [_c.POINTER(Params), _i32, _c.POINTER(_P)]),
+    "ffbf_create_split": (_i32, [_c.POINTER(SplitParams), _i32, _c.POINTER(_P)]),
     "ffbf_destroy": (None, [_P]),
     "ffbf_set_patterns": (_i32, [_P, _P, _P, _u64]),
     "ffbf_set_patterns_lines": (_i32, [_P, _c.c_char_p, _u64]),
@@ -359,6 +373,29 @@ class Matcher:
         self.window, self.k, self.log2_bits = window, k, log2_bits
         self.n_words = 1 << (log2_bits - 5)
         self.n_patterns = 0
+
+    @classmethod
+    def split(cls, window: int, k: int, s: int, log2_cache_bits: int, log2_ext_bits: int, sbox64: Iterable[int],
+              device: int = 0) -> "Matcher":
+        """A context of the CPU binary's split filter (ffbf_create_split):
+        64-bit sbox hash, the first s of k functions in a 2^c-bit cache part,
+        the rest in a 2^e-bit external part, lines hashed separately."""
+        sb = np.asarray(list(sbox64) if not isinstance(sbox64, np.ndarray) else sbox64, dtype=np.uint64)
+        if sb.shape != (256,):
+            raise ValueError("sbox must have 256 entries")
+        p = SplitParams()
+        p.window, p.k, p.s, p.flags = window, k, s, 0
+        p.log2_cache_bits, p.log2_ext_bits = log2_cache_bits, log2_ext_bits
+        for i in range(256):
+            p.sbox[i] = int(sb[i])
+        h = _c.c_void_p(0)
+        _check(_lib.ffbf_create_split(_c.byref(p), device, _c.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.window, self.k, self.log2_bits = window, k, 0
+        self.n_words = ((1 << log2_cache_bits) + (1 << log2_ext_bits)) // 32
+        self.n_patterns = 0
+        return self
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

@@ -322,6 +322,7 @@ int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch *
     for (uint32_t i = 0; i < n_filters; i++) {
         const ffbf_ctx *c = filters[i];
         if (!c) return fail(FFBF_EINVAL, "null filter");
+        if (c->split) return fail(FFBF_EINVAL, "batches take contexts of ffbf_create (not the split filter)");
         if (!c->built || !c->patterns) return fail(FFBF_ESTATE, "every filter must have patterns and a built filter");
         if (c->device != f0->device || c->params.window != f0->params.window || c->params.k != f0->params.k ||
             std::memcmp(c->params.sbox, f0->params.sbox, sizeof f0->params.sbox) != 0)

@@ -103,8 +103,65 @@ uint32_t l2_fold_log2(ffbf_ctx *c)
 // the cheapest of the LDS fold, F probed directly, and (F beyond one XCD's
 // L2) an L2-sized fold, by an expected-cost model over the analytic fills.
 // The folds are computed on the stream; nothing is read back.
+// Split filter: an LDS fold of the cache part (tested by the functions
+// below s) or F probed directly, its cache part first.  The same cost model
+// as below, over the cache part's expected fill.
+int choose_mode_split(ffbf_ctx *c)
+{
+    const ffbf::SplitMap &m = c->smap;
+    const uint32_t S = m.s;
+    if (c->mode_req == ffbf::MODE_LDS || c->mode_req == ffbf::MODE_GFOLD)
+        return fail(FFBF_EINVAL, "split filter: scan mode must be -1, 0 or 1");
+    if (c->mode_req == ffbf::MODE_FOLD && S == 0)
+        return fail(FFBF_EINVAL, "split filter with s = 0 has no cache part to fold");
+    const uint64_t cache_words = (uint64_t)m.cache / 32u;
+    const uint32_t max_words = ffbf::max_fold_words(c->device, true);
+    uint32_t words = (uint32_t)std::min<uint64_t>(max_words, cache_words);
+    if (c->fold_log2_req) {
+        if (c->fold_log2_req < 7 || (1ull << (c->fold_log2_req - 5)) > max_words ||
+            (1ull << (c->fold_log2_req - 5)) > cache_words)
+            return fail(FFBF_EINVAL, "fold size out of range");
+        words = 1u << (c->fold_log2_req - 5);
+    }
+    // s bits per hashed pattern land in the cache part (after a filter load
+    // the whole filter's exact fill stands in for it)
+    const double fc = c->f_set_bits >= 0
+                          ? std::min(1.0, (double)c->f_set_bits / ((double)c->f_words * 32.0))
+                          : 1.0 - std::exp((double)S * (double)c->n_long * std::log1p(-1.0 / (double)m.cache));
+    const double ff = 1.0 - std::pow(1.0 - fc, (double)cache_words / (double)words);
+    uint32_t P = c->probes_req ? c->probes_req : (ff < 0.01 ? 1u : (ff < 0.2 ? 2u : 3u));
+    P = std::max(1u, std::min(std::min(P, S), 3u));
+    bool fold = false;
+    if (S > 0 && c->mode_req != ffbf::MODE_GLOBAL) {
+        if (c->mode_req == ffbf::MODE_FOLD) {
+            fold = true;
+        } else {
+            constexpr double kLds = 1.25, kL2 = 3.8, kFar = 18.0, kQueue = 0.8;
+            const double first = (uint64_t)m.cache / 8u <= (4ull << 20) ? kL2 : kFar;
+            const double cost_global = first * (1.0 + fc);
+            const double cost_fold = kLds + std::pow(ff, (double)P) * kQueue + std::pow(ff, (double)S) * first;
+            fold = cost_fold <= cost_global;
+        }
+    }
+    if (fold) {
+        RET_IF(c->fold.ensure(words, "fold"));
+        HIP_TRY(ffbf::launch_fold(c->F.p, cache_words, c->fold.p, words, c->stream));
+        c->mode = ffbf::MODE_FOLD;
+        c->pre_words = words;
+        c->pre_fill = ff;
+        c->probes = P;
+    } else {
+        c->mode = ffbf::MODE_GLOBAL;
+        c->pre_words = (uint32_t)(S > 0 ? cache_words : ((uint64_t)m.emask + 1u) / 32u);
+        c->pre_fill = fc;
+        c->probes = 1;
+    }
+    return FFBF_OK;
+}
+
 int choose_mode(ffbf_ctx *c)
 {
+    if (c->split) return choose_mode_split(c);
     const uint32_t max_words = ffbf::max_fold_words(c->device);
     const uint32_t k = c->params.k;
     int st = FFBF_OK;
@@ -196,6 +253,7 @@ int choose_mode(ffbf_ctx *c)
 
 uint32_t scan_block(ffbf_ctx *c)
 {
+    if (c->split) return c->mode == ffbf::MODE_FOLD ? 1024u : 256u;
     if (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) return c->block_req == 512u ? 512u : 1024u;
     return 256u;
 }
@@ -274,6 +332,17 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
     a.pre_log2w = pow2_at_least(c->pre_words);
     a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
+    if (c->split) {
+        a.method = ffbf::METHOD_SPLIT;
+        a.split = c->smap;
+        a.pre_exact = 0;
+        if (c->mode == ffbf::MODE_GLOBAL) {
+            // F's bit of function 0: the cache part, or the external part
+            // when no function addresses the cache part
+            a.pre = c->smap.s > 0 ? c->F.p : c->F.p + c->smap.cache / 32u;
+            a.pre_mask = c->smap.s > 0 ? c->smap.cmask : c->smap.emask;
+        }
+    }
     HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
     if (mark_end) RET_IF(record(c, EV_S2));  // the scan kernel alone ends here
     HIP_TRY(ffbf::launch_cand_compact(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
@@ -314,6 +383,8 @@ int enqueue_pass(ffbf_ctx *c, bool fresh)
     a.slot_chain = c->table.p + c->table_size;
     a.chain_next = c->chain_next.p;
     a.table_mask = c->table_size - 1u;
+    a.method = c->split ? ffbf::METHOD_SPLIT : ffbf::METHOD_PAIR;
+    a.split = c->smap;
     HIP_TRY(ffbf::launch_pattern_pass(a, c->n_cu, c->stream));
     RET_IF(record(c, EV_P1));
     return FFBF_OK;
@@ -451,14 +522,13 @@ const char *ffbf_last_error(void) { return ffbf::g_err.c_str(); }
 
 const char *ffbf_version(void) { return "ffbf-amd 0.1 (gfx950)"; }
 
-int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out)
+}  // extern "C"
+
+namespace {
+
+// Context set-up shared by both methods: sp null for the pair method.
+int create_ctx(const ffbf_params *params, const ffbf_split_params *sp, int device, ffbf_ctx **out)
 {
-    if (!params || !out) return fail(FFBF_EINVAL, "null argument");
-    *out = nullptr;
-    if (params->window < 1 || params->window > 64) return fail(FFBF_EINVAL, "window must be 1..64");
-    if (params->k < 1 || params->k > 5) return fail(FFBF_EINVAL, "k must be 1..5");
-    if (params->log2_bits < 10 || params->log2_bits > 32) return fail(FFBF_EINVAL, "log2_bits must be 10..32");
-    if (params->flags != 0) return fail(FFBF_EINVAL, "flags must be 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         (void)hipGetLastError();
@@ -469,21 +539,43 @@ int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out)
     if (!c) return fail(FFBF_ENOMEM, "context allocation");
     c->params = *params;
     c->device = device;
+    // the sbox as the kernels read it: 256 words, or for the split method
+    // 256 (lo, hi) pairs with bytes >= 0x80 mapped to entry 0 (ffbf.cpp:127-130)
+    std::vector<uint32_t> sb(params->sbox, params->sbox + 256);
+    if (sp) {
+        c->split = true;
+        c->split_params = *sp;
+        c->smap.s = sp->s;
+        c->smap.cache = 1u << sp->log2_cache_bits;
+        c->smap.cmask = c->smap.cache - 1u;
+        c->smap.emask = (uint32_t)((1ull << sp->log2_ext_bits) - 1u);
+        sb.assign(512, 0u);
+        for (int b = 0; b < 256; b++) {
+            const uint64_t v = sp->sbox[b >= 0x80 ? 0 : b];
+            sb[2 * b] = (uint32_t)v;
+            sb[2 * b + 1] = (uint32_t)(v >> 32);
+        }
+    }
     int st = FFBF_OK;
     auto init = [&]() -> int {
         HIP_TRY(hipSetDevice(device));
         HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
         HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (int i = 0; i < EV_N; i++) HIP_TRY(hipEventCreate(&c->ev[i]));
-        c->mask = params->log2_bits >= 32 ? 0xFFFFFFFFu : ((1u << params->log2_bits) - 1u);
-        c->f_words = 1ull << (params->log2_bits - 5);
+        if (sp) {
+            c->mask = 0xFFFFFFFFu;  // unused: indices are placed by smap
+            c->f_words = ((1ull << sp->log2_cache_bits) + (1ull << sp->log2_ext_bits)) / 32u;
+        } else {
+            c->mask = params->log2_bits >= 32 ? 0xFFFFFFFFu : ((1u << params->log2_bits) - 1u);
+            c->f_words = 1ull << (params->log2_bits - 5);
+        }
         RET_IF(c->F.ensure(c->f_words, "forward filter"));
         RET_IF(c->R.ensure(c->f_words, "reverse filter"));
-        RET_IF(c->sbox.ensure(256, "sbox"));
+        RET_IF(c->sbox.ensure(sb.size(), "sbox"));
         RET_IF(c->counters.ensure(ffbf::C_COUNT, "counters"));
         HIP_TRY(hipHostMalloc((void **)&c->h_counters, ffbf::C_COUNT * sizeof(unsigned long long),
                               hipHostMallocDefault));
-        HIP_TRY(hipMemcpyAsync(c->sbox.p, params->sbox, 256 * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpy(c->sbox.p, sb.data(), sb.size() * 4, hipMemcpyHostToDevice));
         HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
         HIP_TRY(ffbf::launch_fill32(c->R.p, c->f_words, 0u, c->stream));
         RET_IF(clear_counters(c, 0, ffbf::C_COUNT));
@@ -499,6 +591,37 @@ int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out)
     }
     *out = c;
     return FFBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out)
+{
+    if (!params || !out) return fail(FFBF_EINVAL, "null argument");
+    *out = nullptr;
+    if (params->window < 1 || params->window > 64) return fail(FFBF_EINVAL, "window must be 1..64");
+    if (params->k < 1 || params->k > 5) return fail(FFBF_EINVAL, "k must be 1..5");
+    if (params->log2_bits < 10 || params->log2_bits > 32) return fail(FFBF_EINVAL, "log2_bits must be 10..32");
+    if (params->flags != 0) return fail(FFBF_EINVAL, "flags must be 0");
+    return create_ctx(params, nullptr, device, out);
+}
+
+int ffbf_create_split(const ffbf_split_params *sp, int device, ffbf_ctx **out)
+{
+    if (!sp || !out) return fail(FFBF_EINVAL, "null argument");
+    *out = nullptr;
+    if (sp->window < 1 || sp->window > 64) return fail(FFBF_EINVAL, "window must be 1..64");
+    if (sp->k < 1 || sp->k > (uint32_t)ffbf::kSplitMaxK) return fail(FFBF_EINVAL, "k must be 1..11");
+    if (sp->s > sp->k) return fail(FFBF_EINVAL, "s must be 0..k");
+    if (sp->log2_cache_bits < 5 || sp->log2_cache_bits > 31 || sp->log2_ext_bits < 5 || sp->log2_ext_bits > 31)
+        return fail(FFBF_EINVAL, "log2_cache_bits and log2_ext_bits must be 5..31");
+    if (sp->flags != 0) return fail(FFBF_EINVAL, "flags must be 0");
+    ffbf_params p{};
+    p.window = sp->window;
+    p.k = sp->k;
+    return create_ctx(&p, sp, device, out);
 }
 
 void ffbf_destroy(ffbf_ctx *c)
@@ -560,8 +683,12 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
     HIP_TRY(hipMemcpyAsync(c->off.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
     RET_IF(clear_counters(c, ffbf::C_HASHED, 1));
     // prefix hash pairs only (F untouched until ffbf_build)
-    HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->mask, nullptr,
-                               c->pat_hash.p, c->counters.p, c->stream));
+    if (c->split)
+        HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->smap,
+                                         nullptr, c->pat_hash.p, c->counters.p, c->stream));
+    else
+        HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->mask,
+                                   nullptr, c->pat_hash.p, c->counters.p, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->n_pat = n;
     c->n_long = n_long;
@@ -599,8 +726,12 @@ int ffbf_build(ffbf_ctx *c)
     RET_IF(record(c, EV_B0));
     HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
     RET_IF(clear_counters(c, ffbf::C_HASHED, 1));
-    HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k, c->mask,
-                               c->F.p, c->pat_hash.p, c->counters.p, c->stream));
+    if (c->split)
+        HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
+                                         c->smap, c->F.p, c->pat_hash.p, c->counters.p, c->stream));
+    else
+        HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
+                                   c->mask, c->F.p, c->pat_hash.p, c->counters.p, c->stream));
     RET_IF(record(c, EV_B1));
     c->f_set_bits = -1;
     RET_IF(choose_mode(c));
@@ -823,7 +954,7 @@ int ffbf_filter_load(ffbf_ctx *c, const uint32_t *deltas, uint64_t n)
     uint32_t prev = 0;
     for (uint64_t i = 0; i < n; i++) {
         prev += deltas[i];
-        if ((prev & c->mask) != prev) return fail(FFBF_EINVAL, "bit index beyond the filter size");
+        if ((uint64_t)prev >= c->f_words * 32u) return fail(FFBF_EINVAL, "bit index beyond the filter size");
         h[prev >> 5] |= 1u << (prev & 31u);
     }
     HIP_TRY(hipMemcpyAsync(c->F.p, h.data(), c->f_words * 4, hipMemcpyHostToDevice, c->stream));

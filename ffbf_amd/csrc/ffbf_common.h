@@ -49,6 +49,69 @@ __device__ __forceinline__ uint32_t rev_idx(int j, uint32_t h1, uint32_t h2)
     }
 }
 
+// Split method (src/rabin-karp/ffbf.cpp): forward functions of the 64-bit
+// hash's halves (getHashValue, :135-159) and reverse ones (getHashValue_2,
+// :161-185), before placement.
+__device__ __forceinline__ uint32_t split_fwd_fn(int j, uint32_t h1, uint32_t h2)
+{
+    switch (j) {
+    case 0: return h1;
+    case 1: return h2;
+    case 2: return rotl(h1, 16);
+    case 3: return rotl(h2, 16);
+    case 4: return h1 + h2;
+    case 5: return h1 + (h2 << 1);
+    case 6: return h1 + (h2 << 2);
+    case 7: return h1 + 3u * h2;
+    case 8: return h1 + 5u * h2;
+    case 9: return h1 + 6u * h2;
+    default: return h1 + 7u * h2;
+    }
+}
+
+__device__ __forceinline__ uint32_t split_rev_fn(int j, uint32_t h1, uint32_t h2)
+{
+    switch (j) {
+    case 0: return h1 + (h2 << 1);
+    case 1: return h1 + (h2 << 2);
+    case 2: return h1 + 3u * h2;
+    case 3: return h1 + 5u * h2;
+    case 4: return h1 + 6u * h2;
+    case 5: return h1 + 7u * h2;
+    case 6: return h1;
+    case 7: return h2;
+    case 8: return rotl(h1, 16);
+    case 9: return rotl(h2, 16);
+    default: return h1 + h2;
+    }
+}
+
+__device__ __forceinline__ uint32_t split_place(int j, uint32_t x, const SplitMap &m)
+{
+    return (uint32_t)j < m.s ? (x & m.cmask) : (x & m.emask) + m.cache;
+}
+
+// 64-bit rotation left of (lo, hi) by r mod 64, as two funnel shifts
+// (v_alignbit_b32 takes the high word first); r is a constant after
+// unrolling wherever speed matters.
+__device__ __forceinline__ void rol64(uint32_t lo, uint32_t hi, uint32_t r, uint32_t &olo, uint32_t &ohi)
+{
+    r &= 63u;
+    if (r & 32u) {
+        const uint32_t t = lo;
+        lo = hi;
+        hi = t;
+    }
+    r &= 31u;
+    if (r == 0u) {
+        olo = lo;
+        ohi = hi;
+        return;
+    }
+    olo = __builtin_amdgcn_alignbit(lo, hi, 32u - r);
+    ohi = __builtin_amdgcn_alignbit(hi, lo, 32u - r);
+}
+
 __device__ __forceinline__ uint32_t bit_of(const uint32_t *v, uint32_t b) { return (v[b >> 5] >> (b & 31u)) & 1u; }
 
 // Bit x (taken modulo 32*2^log2_words) of a filter held in LDS: word index by
@@ -140,6 +203,29 @@ __device__ __forceinline__ void feed_forward(bool hit, uint32_t h1, uint32_t h2,
             const uint64_t same = __ballot(hit && w == w0) & act;
             uint32_t bits = 0;
             for (uint64_t m = same; m; m &= m - 1) bits |= __builtin_amdgcn_readlane(bit, __builtin_ctzll(m));
+            if (lane == leader) atomicOr(&R[w0], bits);
+            act &= ~same;
+        }
+    }
+}
+
+// The same for the split method: k (<= 11) reverse bits placed in the cache
+// or external part (setBitsInFilter_2, ffbf.cpp:205-212).
+__device__ __forceinline__ void feed_forward_split(bool hit, uint32_t h1, uint32_t h2, uint32_t k, const SplitMap &m,
+                                                   uint32_t *R)
+{
+    const int lane = lane_id();
+    for (int j = 0; j < kSplitMaxK; j++) {
+        if (j >= (int)k) break;
+        const uint32_t idx = split_place(j, split_rev_fn(j, h1, h2), m);
+        const uint32_t w = idx >> 5, bit = 1u << (idx & 31u);
+        uint64_t act = __ballot(hit);
+        while (act) {
+            const int leader = __builtin_ctzll(act);
+            const uint32_t w0 = __builtin_amdgcn_readlane(w, leader);
+            const uint64_t same = __ballot(hit && w == w0) & act;
+            uint32_t bits = 0;
+            for (uint64_t mm = same; mm; mm &= mm - 1) bits |= __builtin_amdgcn_readlane(bit, __builtin_ctzll(mm));
             if (lane == leader) atomicOr(&R[w0], bits);
             act &= ~same;
         }

@@ -114,7 +114,11 @@ uint32_t pow2_at_least(uint64_t n);
 }  // namespace ffbf
 
 struct ffbf_ctx {
-    ffbf_params params{};
+    ffbf_params params{};  // window and k hold for both methods
+    // the CPU binary's split filter (ffbf_create_split)
+    bool split = false;
+    ffbf_split_params split_params{};
+    ffbf::SplitMap smap{};
     int device = 0;
     int n_cu = 0;
     hipStream_t stream = nullptr;

@@ -45,6 +45,20 @@ enum ScanMode : uint32_t {
     MODE_GFOLD = 3,   // OR-fold of F in global memory, small enough to stay in L2, then F
 };
 
+// Hash method of a context.  PAIR: the GPU reference's two 32-bit rolling
+// hashes and one power-of-two filter (greptest.cu).  SPLIT: the CPU binary's
+// 64-bit hash, filter in a cache part and an external part, hashing
+// restarted at every newline and bytes >= 0x80 hashed as 0 (ffbf.cpp).
+enum HashMethod : uint32_t { METHOD_PAIR = 0, METHOD_SPLIT = 1 };
+
+// Placement of the split method's bit indices (ffbf.cpp:154-158,
+// 180-184): function j < s keeps x & cmask (cache part, [0, cache)); the
+// others (x & emask) + cache (external part).
+struct SplitMap {
+    uint32_t s, cmask, emask, cache;
+};
+constexpr int kSplitMaxK = 11;  // getHashValue has 11 functions
+
 struct ScanArgs {
     const uint8_t *text;
     uint64_t n;
@@ -69,6 +83,8 @@ struct ScanArgs {
     uint32_t wave0;         // region slot of the launch's first wave
     uint32_t region_end;    // slots [wave0, region_end) are the launch's; waves beyond only spill
     uint64_t spill_base;    // first entry after all region slots
+    uint32_t method;        // HashMethod; SPLIT: sbox is 256 (lo, hi) pairs, mask unused
+    SplitMap split;
 };
 
 // Batched scan: many texts (concatenated in one device buffer), each routed
@@ -155,6 +171,8 @@ struct PassArgs {
     uint32_t *slot_chain;
     uint32_t *chain_next;
     uint32_t table_mask;
+    uint32_t method;  // HashMethod
+    SplitMap split;
 };
 
 // A run of u32 words to clear (a few thousand words each, so that many
@@ -170,6 +188,11 @@ hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s
 hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
                         uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, unsigned long long *counters,
                         hipStream_t s);
+// The split method's build: 64-bit prefix hashes from a 256-entry (lo, hi)
+// sbox, k bits placed by m (F null: hashes only).
+hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
+                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash,
+                              unsigned long long *counters, hipStream_t s);
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s);
 hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
@@ -204,8 +227,9 @@ hipError_t sort_temp_bytes(uint64_t n, int end_bit, size_t *bytes);
 hipError_t sort_matches(MatchRec *m, uint64_t n, uint64_t *keys_a, uint64_t *keys_b, void *temp, size_t temp_bytes,
                         int end_bit, hipStream_t s);
 
-// Largest fold (in 32-bit words) the scan kernel can hold in LDS on this device.
-uint32_t max_fold_words(int device);
+// Largest fold (in 32-bit words) the scan kernel can hold in LDS on this
+// device (split: beside the split method's 2 KiB sbox).
+uint32_t max_fold_words(int device, bool split = false);
 // Whether a templated fast scan exists for this window length.
 bool scan_has_fast_path(uint32_t W);
 

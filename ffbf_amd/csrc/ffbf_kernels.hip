@@ -8,6 +8,7 @@
 //   k_verify        the exact step the reference leaves to `grep -Ff`
 //                   (src/rabin-karp/README:49-52)
 //   k_fold          OR-fold of F used by the scan as an exact-superset prefilter
+//   k_build_split   Filter::operator<< of the CPU binary  src/rabin-karp/ffbf.cpp:268-283
 #include "ffbf_common.h"
 
 namespace ffbf {
@@ -68,6 +69,47 @@ __global__ __launch_bounds__(256) void k_build(const uint8_t *bytes, const uint6
     if (hashed) atomicAdd(counters + C_HASHED, (unsigned long long)hashed);
 }
 
+// The split method's build (Filter::operator<<, ffbf.cpp:268-283): one lane
+// per pattern hashes its W-byte prefix with the 64-bit loop form
+// h = rol64(h,1) ^ S[c] (the ring update of hash_rot_sbox_pre_2.h:41-50 before
+// the window is full; the sbox has bytes >= 0x80 mapped to entry 0) and sets
+// k bits, each in the cache or external part (setBitsInFilter,
+// ffbf.cpp:196-203).  Hash halves are kept as (h1, h2) = (low, high).
+__global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n,
+                                                     const uint2 *sbox, uint32_t W, uint32_t k, SplitMap m,
+                                                     uint32_t *F, uint2 *pat_hash, unsigned long long *counters)
+{
+    __shared__ uint2 sb[256];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) sb[i] = sbox[i];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t hashed = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t a0 = off[i], a1 = off[i + 1];
+        uint32_t lo = 0, hi = 0;
+        if (a1 - a0 >= W) {
+            const uint8_t *p = bytes + a0;
+            for (uint32_t j = 0; j < W; j++) {
+                const uint2 v = sb[p[j]];
+                uint32_t rl, rh;
+                rol64(lo, hi, 1u, rl, rh);
+                lo = rl ^ v.x;
+                hi = rh ^ v.y;
+            }
+            if (F) {
+                for (int j = 0; j < kSplitMaxK; j++) {
+                    if (j >= (int)k) break;
+                    const uint32_t idx = split_place(j, split_fwd_fn(j, lo, hi), m);
+                    atomicOr(&F[idx >> 5], 1u << (idx & 31u));
+                }
+            }
+            hashed++;
+        }
+        pat_hash[i] = make_uint2(lo, hi);
+    }
+    if (hashed) atomicAdd(counters + C_HASHED, (unsigned long long)hashed);
+}
+
 // OR-fold of F onto fold_words words: fold bit x is set iff some F bit
 // congruent to x mod fold_words*32 is set, so a window whose k forward bits are
 // set in F has its k folded bits set too (exact superset test).
@@ -85,6 +127,9 @@ __global__ void k_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint
 // hash pair against R.  A wave owns 64 consecutive ids (i0 a multiple of 64),
 // so its ballot is two whole bitmap words; survivors are compacted with a
 // wave prefix count and indexed by prefix-hash key for verification.
+// The split method tests its k (<= 11) reverse functions, placed in the
+// cache or external part (checkInFilter_2, ffbf.cpp:241-266).
+template <bool SPLIT = false>
 __device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
 {
     const int lane = lane_id();
@@ -92,7 +137,15 @@ __device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
     const uint64_t i = i0 + (uint64_t)lane;
     bool ok = false;
     uint2 h = make_uint2(0u, 0u);
-    if (i < a.n) {
+    if (i < a.n && SPLIT) {
+        ok = a.off[i + 1] - a.off[i] >= a.W;
+        h = a.pat_hash[i];
+        for (int j = 0; j < kSplitMaxK; j++) {
+            if (j >= (int)a.k || !ok) break;
+            const uint32_t ix = split_place(j, split_rev_fn(j, h.x, h.y), a.split);
+            ok = (a.R[ix >> 5] >> (ix & 31u)) & 1u;
+        }
+    } else if (i < a.n) {
         ok = a.off[i + 1] - a.off[i] >= a.W;
         h = a.pat_hash[i];
         uint32_t words[5], idx[5];
@@ -130,11 +183,12 @@ __device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
     }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
 {
     const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) pass_wave(a, i0);
+    for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) pass_wave<SPLIT>(a, i0);
 }
 
 // The pattern passes of several filters in one launch: wave_start[f] is the
@@ -157,7 +211,7 @@ __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filt
         const PassArgs &a = filters[lo];
         const uint64_t i0 = (gw - wave_start[lo]) << 6;
         if (active[lo]) {
-            pass_wave(a, i0);
+            pass_wave<false>(a, i0);
         } else {
             const uint64_t n_words = (a.n + 31u) / 32u, w = i0 >> 5;
             const int lane = lane_id();
@@ -239,6 +293,18 @@ hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, c
     return hipGetLastError();
 }
 
+hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
+                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash,
+                              unsigned long long *counters, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    hipLaunchKernelGGL(k_build_split, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, (const uint2 *)sbox64,
+                       W, k, m, F, pat_hash, counters);
+    return hipGetLastError();
+}
+
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s)
 {
     uint64_t blocks = ((uint64_t)fold_words + 255u) / 256u;
@@ -253,7 +319,10 @@ hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s)
     uint64_t blocks = (a.n + 255u) / 256u;
     const uint64_t cap = (uint64_t)n_cu * 8u;
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(k_pattern_pass, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    if (a.method == METHOD_SPLIT)
+        hipLaunchKernelGGL(k_pattern_pass<true>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_pattern_pass<false>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

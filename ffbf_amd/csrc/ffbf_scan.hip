@@ -25,15 +25,16 @@ namespace ffbf {
 
 namespace {
 
-template <int WT, bool LDSPRE, int BLOCK, int P>
+template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
 {
     using QG = QueueGeom<BLOCK>;
+    constexpr uint32_t kSboxWords = SPLIT ? 512u : 256u;  // split: 256 (lo, hi) pairs
     extern __shared__ __align__(16) uint32_t s_pre[];
-    __shared__ uint32_t s_sbox[256];
+    __shared__ __align__(8) uint32_t s_sbox[kSboxWords];
     __shared__ uint4 s_q[BLOCK / 64][QG::cap];
 
-    for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) s_sbox[i] = a.sbox[i];
+    for (uint32_t i = threadIdx.x; i < kSboxWords; i += BLOCK) s_sbox[i] = a.sbox[i];
     if constexpr (LDSPRE) {
         const uint4 *src = (const uint4 *)a.pre;
         uint4 *dst = (uint4 *)s_pre;
@@ -62,38 +63,47 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
     for (; it < iters; it += n_waves) {
         const uint64_t u = it * kIterUnits + (uint64_t)lane_id();
         uint32_t h1[kUnit], h2[kUnit];
+        uint32_t line_ok = 0xFFFFu;
         if constexpr (WT > 0) {
             // issue the next iteration's load before hashing this one
             const uint64_t nit = it + n_waves;
             uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
             if (nit < iters) nxt = load_chunk(base, lead, a.n, nit * kIterUnits + (uint64_t)lane_id());
-            hash_unit<(WT > 0 ? WT : 1)>(cur, s_sbox, h1, h2);
+            if constexpr (SPLIT)
+                line_ok = hash_unit64<(WT > 0 ? WT : 1)>(cur, (const uint2 *)s_sbox, h1, h2);
+            else
+                hash_unit<(WT > 0 ? WT : 1)>(cur, s_sbox, h1, h2);
             cur = nxt;
         } else {
-            hash_unit_generic(base, lead, a.n, u, W, s_sbox, h1, h2);
+            if constexpr (SPLIT)
+                line_ok = hash_unit_generic64(base, lead, a.n, u, W, (const uint2 *)s_sbox, h1, h2);
+            else
+                hash_unit_generic(base, lead, a.n, u, W, s_sbox, h1, h2);
         }
 
         // window offset of slot b is 16u + b - lead
-        test_and_queue<LDSPRE, BLOCK, P>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos, a.off_base, q, qn, pend);
+        test_and_queue<LDSPRE, BLOCK, P, SPLIT>(a, s_pre, h1, h2, (int64_t)(u * 16u) - (int64_t)lead, npos,
+                                                a.off_base, q, qn, pend, line_ok);
     }
     if (qn) {
         wave_sync();
-        drain<LDSPRE, P>(q, qn, a, s_pre, pend);
+        drain<LDSPRE, P, SPLIT>(q, qn, a, s_pre, pend);
     }
     finish_pending(pend, a);
 }
 
-template <int WT, bool LDSPRE, int BLOCK, int P>
+template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false>
 hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
 {
     const size_t dyn = LDSPRE ? (size_t)a.pre_words * 4u : 0;
     if (dyn > 0) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_scan<WT, LDSPRE, BLOCK, P>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan<WT, LDSPRE, BLOCK, P, SPLIT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
     }
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<WT, LDSPRE, BLOCK, P>, BLOCK, dyn);
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<WT, LDSPRE, BLOCK, P, SPLIT>, BLOCK, dyn);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const uint64_t W = WT ? (uint64_t)WT : a.W;
@@ -103,8 +113,22 @@ hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
     const uint64_t need = (iters + (BLOCK / 64) - 1) / (BLOCK / 64);
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((k_scan<WT, LDSPRE, BLOCK, P>), dim3((uint32_t)grid), dim3(BLOCK), dyn, s, a);
+    hipLaunchKernelGGL((k_scan<WT, LDSPRE, BLOCK, P, SPLIT>), dim3((uint32_t)grid), dim3(BLOCK), dyn, s, a);
     return hipGetLastError();
+}
+
+// Split method: the cache-part fold in LDS (1024-thread blocks, P <= 3 bits
+// up front) or F's first bit read directly (256-thread blocks).
+template <int WT>
+hipError_t launch_scan_split_w(const ScanArgs &a, uint32_t mode, int n_cu, hipStream_t s)
+{
+    if (mode == MODE_FOLD) {
+        const uint32_t P = a.P < 1u ? 1u : (a.P > 3u ? 3u : a.P);
+        if (P == 1) return launch_scan_t<WT, true, 1024, 1, true>(a, n_cu, s);
+        if (P == 2) return launch_scan_t<WT, true, 1024, 2, true>(a, n_cu, s);
+        return launch_scan_t<WT, true, 1024, 3, true>(a, n_cu, s);
+    }
+    return launch_scan_t<WT, false, 256, 1, true>(a, n_cu, s);
 }
 
 // LDS prefilter: 1024-thread blocks (512 where a smaller LDS forces it), P
@@ -148,13 +172,14 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t *arr, uint32_t ma
 
 bool scan_has_fast_path(uint32_t W) { return W == 19 || W == 24 || W == 32; }
 
-uint32_t max_fold_words(int device)
+uint32_t max_fold_words(int device, bool split)
 {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds <= 0)
         lds = 64 * 1024;
-    // static LDS of the 512-thread scan: sbox 1 KiB + 8 queues of 2 KiB
-    const int static_lds = 1024 + 8 * QueueGeom<512>::cap * 16;
+    // static LDS of the 512-thread scan: sbox 1 KiB + 8 queues of 3.75 KiB;
+    // the split method's 1024-thread scan: sbox 2 KiB + 16 queues of 1.875 KiB
+    const int static_lds = split ? 2048 + 16 * QueueGeom<1024>::cap * 16 : 1024 + 8 * QueueGeom<512>::cap * 16;
     uint32_t words = 1u << 15;  // 128 KiB
     while (words > 256u && (int)(words * 4u) + static_lds > lds) words >>= 1;
     return words;
@@ -162,6 +187,10 @@ uint32_t max_fold_words(int device)
 
 hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s)
 {
+    if (a.method == METHOD_SPLIT) {
+        if (mode != MODE_FOLD && mode != MODE_GLOBAL) return hipErrorInvalidValue;
+        return a.W == 19 ? launch_scan_split_w<19>(a, mode, n_cu, s) : launch_scan_split_w<0>(a, mode, n_cu, s);
+    }
     switch (a.W) {
     case 19: return launch_scan_w<19>(a, mode, block, n_cu, s);
     case 24: return launch_scan_w<24>(a, mode, block, n_cu, s);

@@ -146,6 +146,126 @@ static __device__ __forceinline__ void hash_unit_generic(const uint8_t *base, ui
     }
 }
 
+// Bit i (0..3) set where byte i of w is '\n'.  Exact per byte: v's bytes
+// are 0 exactly at newlines, and ((v & 0x7F) + 0x7F) | v has bit 7 clear only
+// for a zero byte, with no carry between bytes; the multiply gathers bits
+// 7, 15, 23, 31 into bits 21..24 (the partial products do not overlap).
+static __device__ __forceinline__ uint32_t nl_nibble(uint32_t w)
+{
+    const uint32_t v = w ^ 0x0A0A0A0Au;
+    const uint32_t z = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+    return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
+}
+
+// Split method: 64-bit hashes of the 16 windows of a lane's unit (low halves
+// in h1, high in h2) from a 256-entry (lo, hi) sbox whose entries 0x80..0xFF
+// equal entry 0 (ffbf.cpp:124-133), one full window plus 15 rolling slides
+// (hash_rot_sbox_pre_2.h:41-50: h = rol64(h,1) ^ S[in] ^ rol64(S[out], W)).
+// Returns the mask of windows that hold no newline: only those are tested,
+// as the reference restarts the hash at every '\n' (ffbf.cpp:342-358) and a
+// hash is tested once W bytes followed the restart.
+template <int W>
+static __device__ __forceinline__ uint32_t hash_unit64(const uint4 own, const uint2 *sb, uint32_t (&h1)[kUnit],
+                                                       uint32_t (&h2)[kUnit])
+{
+    constexpr int NB = Chunks<W>::NB;
+    constexpr int NCH = Chunks<W>::NCH;
+    static_assert(NCH <= 4, "the newline mask spans 64 bytes");
+    uint32_t lo[NCH * 16], hi[NCH * 16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int d = j >> 2;
+        const uint32_t w = d == 0 ? own.x : d == 1 ? own.y : d == 2 ? own.z : own.w;
+        const uint2 v = sb[(w >> (8 * (j & 3))) & 0xFFu];
+        lo[j] = v.x;
+        hi[j] = v.y;
+    }
+#pragma unroll
+    for (int j = 16; j < NB; j++) {
+        lo[j] = lane_rol1(lo[j - 16]);
+        hi[j] = lane_rol1(hi[j - 16]);
+    }
+    uint32_t xl = lo[W - 1], xh = hi[W - 1];
+#pragma unroll
+    for (int j = 0; j + 1 < W - 1; j += 2) {
+        uint32_t al, ah, bl, bh;
+        rol64(lo[j], hi[j], (uint32_t)(W - 1 - j), al, ah);
+        rol64(lo[j + 1], hi[j + 1], (uint32_t)(W - 2 - j), bl, bh);
+        xl = xor3(xl, al, bl);
+        xh = xor3(xh, ah, bh);
+    }
+    if constexpr (((W - 1) & 1) != 0) {
+        uint32_t al, ah;
+        rol64(lo[W - 2], hi[W - 2], 1u, al, ah);
+        xl ^= al;
+        xh ^= ah;
+    }
+    h1[0] = xl;
+    h2[0] = xh;
+#pragma unroll
+    for (int b = 1; b < kUnit; b++) {
+        uint32_t rl, rh, ol, oh;
+        rol64(xl, xh, 1u, rl, rh);
+        rol64(lo[b - 1], hi[b - 1], (uint32_t)W, ol, oh);
+        xl = xor3(rl, lo[b + W - 1], ol);
+        xh = xor3(rh, hi[b + W - 1], oh);
+        h1[b] = xl;
+        h2[b] = xh;
+    }
+    // newlines in the bytes behind the 16 windows: own unit, then the units
+    // of the lanes above; bit i of `bad` marks a newline in [i, i+W)
+    uint32_t t = nl_nibble(own.x) | (nl_nibble(own.y) << 4) | (nl_nibble(own.z) << 8) | (nl_nibble(own.w) << 12);
+    uint64_t m = t;
+#pragma unroll
+    for (int c = 1; c < NCH; c++) {
+        t = lane_rol1(t);
+        m |= (uint64_t)t << (16 * c);
+    }
+    uint64_t bad = m;
+    int cov = 1;  // bad covers shifts [0, cov)
+#pragma unroll
+    for (; 2 * cov <= W; cov *= 2) bad |= bad >> cov;
+    if (cov < W) bad |= bad >> (W - cov);
+    return ~(uint32_t)bad & 0xFFFFu;
+}
+
+// Run-time W for the split method: bytes fetched one at a time.
+static __device__ __forceinline__ uint32_t hash_unit_generic64(const uint8_t *base, uint64_t lead, uint64_t n,
+                                                               uint64_t u, uint32_t W, const uint2 *sb,
+                                                               uint32_t (&h1)[kUnit], uint32_t (&h2)[kUnit])
+{
+    const uint64_t q0 = u * 16u;
+    uint32_t xl = 0, xh = 0;
+    int last_nl = -1;  // position of the last newline seen, relative to q0
+    for (uint32_t j = 0; j < W; j++) {
+        const uint32_t c = byte_at(base, lead, n, q0 + j);
+        if (c == 10u) last_nl = (int)j;
+        const uint2 v = sb[c];
+        uint32_t rl, rh;
+        rol64(xl, xh, 1u, rl, rh);
+        xl = rl ^ v.x;
+        xh = rh ^ v.y;
+    }
+    uint32_t ok = last_nl < 0 ? 1u : 0u;
+    h1[0] = xl;
+    h2[0] = xh;
+#pragma unroll
+    for (int b = 1; b < kUnit; b++) {
+        const uint32_t ci = byte_at(base, lead, n, q0 + (uint64_t)b + W - 1);
+        if (ci == 10u) last_nl = b + (int)W - 1;
+        const uint2 si = sb[ci], so = sb[byte_at(base, lead, n, q0 + (uint64_t)b - 1)];
+        uint32_t rl, rh, ol, oh;
+        rol64(xl, xh, 1u, rl, rh);
+        rol64(so.x, so.y, W, ol, oh);
+        xl = rl ^ si.x ^ ol;
+        xh = rh ^ si.y ^ oh;
+        h1[b] = xl;
+        h2[b] = xh;
+        ok |= (last_nl < b ? 1u : 0u) << b;
+    }
+    return ok;
+}
+
 // A lane's candidate not yet appended.  Appends go out in bulk: a wave
 // flushes all its held candidates when a new hit lands on a lane that still
 // holds one, and at the end of its work.  A flush writes into the wave's own
@@ -218,7 +338,7 @@ static __device__ __forceinline__ void stage_cand(PendingCand &pc, bool hit, uin
 // Finishes up to 64 queued windows, one per lane: the prefilter bits not yet
 // tested, then (for a fold) the k forward bits of F, then feed-forward and
 // candidate staging.  Called by the whole wave.
-template <bool LDSPRE, int P>
+template <bool LDSPRE, int P, bool SPLIT = false>
 static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const ScanArgs &a, const uint32_t *s_pre,
                                              PendingCand &pend)
 {
@@ -227,6 +347,34 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
     if (act) e = q[lane_id()];
     const uint32_t h1 = e.z, h2 = e.w;
     uint32_t ok = act ? 1u : 0u;
+    if constexpr (SPLIT) {
+        // the fold covers the cache part: its bits P..s-1, then F (the
+        // global prefilter already was F's bit 0), one bit for every window
+        // still passing and the rest only where it is set
+        if constexpr (LDSPRE) {
+            for (int j = P; j < kSplitMaxK; j++) {
+                if (j >= (int)a.split.s) break;
+                ok &= lds_bit(s_pre, split_fwd_fn(j, h1, h2), a.pre_log2w);
+            }
+        }
+        const int j0 = LDSPRE ? 0 : 1;
+        if (ok && j0 < (int)a.k) {
+            const uint32_t i0 = split_place(j0, split_fwd_fn(j0, h1, h2), a.split);
+            ok &= (a.F[i0 >> 5] >> (i0 & 31u)) & 1u;
+        }
+        if (ok) {
+            for (int j = j0 + 1; j < kSplitMaxK; j++) {
+                if (j >= (int)a.k) break;
+                const uint32_t ix = split_place(j, split_fwd_fn(j, h1, h2), a.split);
+                ok &= (a.F[ix >> 5] >> (ix & 31u)) & 1u;
+            }
+        }
+        const bool hit = ok != 0u;
+        const uint64_t off = ((uint64_t)e.y << 32) | e.x;
+        feed_forward_split(hit, h1, h2, a.k, a.split, a.R);
+        stage_cand(pend, hit, off, h1, h2, a);
+        return;
+    }
     if constexpr (LDSPRE) {
         // bits P..k-1 of the LDS prefilter, loads issued back to back
         uint32_t bits[5];
@@ -284,10 +432,11 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
 // survivors onto the wave's LDS queue, draining it 64 at a time.  `tag` is
 // added to the recorded offsets (the offset of a text piece, or the batched
 // scan's text id in the top bits).  Called by the whole wave.
-template <bool LDSPRE, int BLOCK, int P>
+template <bool LDSPRE, int BLOCK, int P, bool SPLIT = false>
 static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const uint32_t *s_pre, const uint32_t (&h1)[kUnit],
                                                       const uint32_t (&h2)[kUnit], int64_t p0, uint64_t npos,
-                                                      uint64_t tag, uint4 *q, uint32_t &qn, PendingCand &pend)
+                                                      uint64_t tag, uint4 *q, uint32_t &qn, PendingCand &pend,
+                                                      uint32_t line_ok = 0xFFFFu)
 {
     using QG = QueueGeom<BLOCK>;
     // the real windows of this lane (0 <= p0 + b < npos) are one run [lo, hi)
@@ -297,6 +446,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
     const uint32_t hi = (uint32_t)(hi64 <= 0 ? 0 : (hi64 < kUnit ? hi64 : kUnit));
     uint32_t vm = hi > lo ? ((0xFFFFu >> (kUnit - hi)) & (0xFFFFu << lo)) : 0u;
     if (lane_id() >= (int)kIterUnits) vm = 0;  // these units belong to the next wave-iteration
+    vm &= line_ok;
     uint32_t pm = 0;
     if constexpr (LDSPRE) {
         // P bits per window, all loads in flight together
@@ -304,7 +454,9 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
 #pragma unroll
         for (int j = 0; j < P; j++)
 #pragma unroll
-            for (int b = 0; b < kUnit; b++) bits[j][b] = lds_bit(s_pre, fwd_idx(j, h1[b], h2[b]), a.pre_log2w);
+            for (int b = 0; b < kUnit; b++)
+                bits[j][b] = lds_bit(s_pre, SPLIT ? split_fwd_fn(j, h1[b], h2[b]) : fwd_idx(j, h1[b], h2[b]),
+                                     a.pre_log2w);
 #pragma unroll
         for (int b = 0; b < kUnit; b++) {
             uint32_t ok = bits[0][b];
@@ -343,7 +495,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         while (qn >= (uint32_t)QG::drain_at) {
             const uint32_t cnt = qn < 64u ? qn : 64u;
             wave_sync();
-            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre, pend);
+            drain<LDSPRE, P, SPLIT>(q + (qn - cnt), cnt, a, s_pre, pend);
             wave_sync();
             qn -= cnt;
         }
@@ -364,7 +516,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         if (qn >= (uint32_t)QG::drain_at) {
             const uint32_t cnt = qn < 64u ? qn : 64u;
             wave_sync();
-            drain<LDSPRE, P>(q + (qn - cnt), cnt, a, s_pre, pend);
+            drain<LDSPRE, P, SPLIT>(q + (qn - cnt), cnt, a, s_pre, pend);
             wave_sync();
             qn -= cnt;
         }

@@ -53,7 +53,8 @@ struct Slot {
 // F and prefilters; fresh R and survivor index.
 int clone_ctx(const ffbf_ctx *src, ffbf_ctx **out)
 {
-    RET_IF(ffbf_create(&src->params, src->device, out));
+    RET_IF(src->split ? ffbf_create_split(&src->split_params, src->device, out)
+                      : ffbf_create(&src->params, src->device, out));
     ffbf_ctx *c = *out;
     c->mode_req = src->mode_req;
     c->fold_log2_req = src->fold_log2_req;

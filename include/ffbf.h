@@ -23,6 +23,7 @@
  *   printpositions            src/gpgpu/greptest.cu:353-398   -> ffbf_write_candidate_lines
  *   printpatterns             src/gpgpu/greptest.cu:400-426   -> ffbf_write_survivors
  *   Filter::save/loadFilterToFile  src/rabin-karp/ffbf.cpp:487-527 -> ffbf_filter_save / ffbf_filter_load
+ *   Filter<hash_rot_sbox_pre_2,19> src/rabin-karp/ffbf.cpp:88-185  -> ffbf_create_split
  *
  * Conventions: every call returns FFBF_OK (0) or a negative status and never
  * exits or throws; ffbf_last_error() gives the calling thread's last message.
@@ -63,6 +64,26 @@ typedef struct ffbf_params {
     uint32_t sbox[256]; /* substitution table of the rolling hash */
 } ffbf_params;
 
+/* The CPU binary's filter (src/rabin-karp/ffbf.cpp, Filter<H, LEN> with
+ * hash_rot_sbox_pre_2): a 64-bit rolling hash whose low and high halves are
+ * h1 and h2 (hash_rot_sbox_pre_2.h:26-27, 41-50); k hash functions
+ * (getHashValue, ffbf.cpp:135-159; reverse getHashValue_2, :161-185), the
+ * first s of them addressing a cache part of 2^c bits and the rest an
+ * external part of 2^e bits placed after it (:154-158); the text hashed line
+ * by line, restarting after every '\n' (:342-358), so windows holding a
+ * newline are never candidates; bytes >= 0x80 hashed as 0 (:124-133).  The
+ * reference's compile-time values are window 19, k 5, s 2, c 24, e 28 and the
+ * table of hash_rot_sbox_pre_2.h:55-120.  Filters hold 2^c + 2^e bits. */
+typedef struct ffbf_split_params {
+    uint32_t window;          /* LEN, 1..64 */
+    uint32_t k;               /* K, 1..11 */
+    uint32_t s;               /* S, 0..k: functions addressing the cache part */
+    uint32_t log2_cache_bits; /* c, 5..31 */
+    uint32_t log2_ext_bits;   /* e, 5..31 */
+    uint32_t flags;           /* must be 0 */
+    uint64_t sbox[256];       /* 64-bit substitution table */
+} ffbf_split_params;
+
 typedef struct ffbf_match {
     uint32_t pattern_id;
     uint32_t reserved;
@@ -92,6 +113,15 @@ const char *ffbf_version(void);
 
 /* Creates a context on HIP device `device`, allocating F and R. */
 int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out);
+
+/* Creates a context of the CPU binary's split filter (Filter<H, LEN>
+ * constructor, ffbf.cpp:88-108).  Every single-text entry point below works
+ * on it unchanged (patterns, build, scan, pattern pass, verification,
+ * outputs, persistence over its 2^c + 2^e bits); batches and servers take
+ * contexts of ffbf_create only.  Scan modes: -1 automatic, 0 (F probed
+ * directly; the cache part first) or 1 (an LDS fold of the cache part,
+ * needs s >= 1). */
+int ffbf_create_split(const ffbf_split_params *params, int device, ffbf_ctx **out);
 void ffbf_destroy(ffbf_ctx *ctx);
 
 /* Loads a pattern set: pattern i is bytes[offsets[i] .. offsets[i+1]);
