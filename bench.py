@@ -16,7 +16,8 @@ inputs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                       [--workload plain|ascii10k-64m|ascii100k-1536m|batch-4096texts-1set|
-                                  batch-2048texts-64sets|serve-8sets|ingest]
+                                  batch-2048texts-64sets|serve-8sets|ingest|
+                                  split-10k-64m-lines|split-1m-2g-lines]
 """
 from __future__ import annotations
 
@@ -37,6 +38,16 @@ WORKLOADS = {
     "plain": ("dna", 32, 4, 28, 1_000_000, 32, 100, 2 << 30, 50_000, 2),
     "ascii10k-64m": ("ascii", 19, 4, 24, 10_000, 20, 40, 64 << 20, 1_000, 1),
     "ascii100k-1536m": ("ascii", 19, 4, 26, 100_000, 20, 64, 1536 << 20, 1_536, 3),
+}
+
+
+# The CPU binary's split filter (ffbf_create_split) at its compile-time shape:
+# W=19, k=5, s=2, cache part 2^24 bits, external part 2^28 bits; ASCII
+# patterns and an ASCII text cut into lines (mean length `line` bytes).
+# name: W, k, s, log2 cache, log2 ext, patterns, lmin, lmax, text bytes, line, plants, seed
+SPLIT_WORKLOADS = {
+    "split-10k-64m-lines": (19, 5, 2, 24, 28, 10_000, 20, 40, 64 << 20, 80, 1_000, 4),
+    "split-1m-2g-lines": (19, 5, 2, 24, 28, 1_000_000, 20, 64, 2 << 30, 80, 50_000, 5),
 }
 
 
@@ -414,12 +425,17 @@ def run_ingest(fa, reps=3):
 
 def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
     """Oracle (C restatement of the reference path) on a bounded text sample."""
-    import numpy as np
     from oracle import Oracle
 
     alpha, W, k, log2, n_pat, lmin, lmax, n_text, plants, seed = wl
+    return _cpu_baseline(Oracle(W, k, log2, sbox), n_pat, pb, po, text, budget_s)
+
+
+def _cpu_baseline(o, n_pat, pb, po, text, budget_s):
+    import numpy as np
+
+    n_text = text.size
     threads = max(1, min(16, os.cpu_count() or 1))
-    o = Oracle(W, k, log2, sbox)
     t0 = time.perf_counter()
     F = o.build(pb, po)
     t_build = time.perf_counter() - t0
@@ -469,6 +485,76 @@ def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
     }
 
 
+def split_inputs(fa, wl):
+    import numpy as np
+
+    W, k, s, c, e, n_pat, lmin, lmax, n_text, line, plants, seed = wl
+    sbox64 = np.random.default_rng(seed).integers(0, 2**64, size=256, dtype=np.uint64)
+    pb, po = fa.gen_patterns(fa.ALPHA_ASCII, seed * 10 + 1, n_pat, lmin, lmax)
+    text = fa.gen_bytes(fa.ALPHA_ASCII, seed * 10 + 2, n_text)
+    rng = np.random.default_rng(seed * 10 + 4)
+    text[rng.integers(0, n_text, size=n_text // line)] = 10
+    fa.plant(seed * 10 + 3, plants, text, pb, po)
+    return sbox64, pb, po, text
+
+
+def run_split(fa, name, steps=10, warmup=2, cpu=True):
+    """The split filter's whole path (build, scan, pattern pass, verify,
+    sorted matches to the host) over a device-resident text, as the plain
+    leg measures the pair method; CPU baseline: the oracle's split method."""
+    import numpy as np
+
+    wl = SPLIT_WORKLOADS[name]
+    W, k, s, c, e, n_pat, lmin, lmax, n_text, line, plants, seed = wl
+    sbox64, pb, po, text = split_inputs(fa, wl)
+    dev = fa.DeviceBuffer.from_host(text)
+    m = fa.Matcher.split(W, k, s, c, e, sbox64)
+    m.set_patterns(pb, po)
+
+    def step():
+        m.build()
+        m.run_device(dev.ptr, n_text)
+        return m.matches()
+
+    for _ in range(warmup):
+        step()
+    m.synchronize()
+    scan_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = step()
+        scan_ms.append(m.timings()["scan_kernel_ms"])
+    m.synchronize()
+    elapsed = time.perf_counter() - t0
+    t = m.timings()
+    t_scan = float(np.mean(scan_ms))
+    achieved = n_text / (t_scan * 1e-3) / 1e9
+    out = {
+        "workload": (f"{name}: split filter W={W} k={k} s={s} cache 2^{c} + external 2^{e} bits, {n_pat} ASCII "
+                     f"patterns ({lmin}-{lmax} B), {n_text / 2**20:.0f} MiB ASCII text in lines of ~{line} B, "
+                     f"{plants} planted"),
+        "value": round(n_text / (elapsed / steps) / 1e9, 3),
+        "unit": "GB/s",
+        "ms_per_step": round(elapsed * 1e3 / steps, 4),
+        "steps": steps,
+        "roofline": {"kernel": "k_scan (split)", "bound": "hbm", "achieved": round(achieved, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "avg_launch_ms": round(t_scan, 4), "algorithmic_bytes": n_text},
+        "scan_mode": {0: "F probed directly (cache part first)", 1: "LDS fold of the cache part"}.get(t["scan_mode"]),
+        "probes": t["probes"],
+        "matches": int(res.size),
+        "candidates": int(m.n_candidates()),
+    }
+    m.close()
+    dev.free()
+    if cpu:
+        from oracle import SplitOracle
+
+        log(f"{name}: cpu baseline ...")
+        out["cpu_baseline"] = _cpu_baseline(SplitOracle(W, k, s, c, e, sbox64), n_pat, pb, po, text, 12.0)
+    return out
+
+
 def traffic_from_profiles(name):
     p = os.path.join(ROOT, "profiles", "scan_traffic.json")
     try:
@@ -485,7 +571,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain",
-                    choices=sorted(WORKLOADS) + ["batch-4096texts-1set", "batch-2048texts-64sets", "serve-8sets", "ingest"])
+                    choices=sorted(WORKLOADS) + sorted(SPLIT_WORKLOADS) +
+                    ["batch-4096texts-1set", "batch-2048texts-64sets", "serve-8sets", "ingest"])
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched, serving and ingest legs of the default run")
     ap.add_argument("--mode", type=int, default=-1, help="scan mode override (-1 auto)")
@@ -529,6 +616,16 @@ def main() -> int:
                           "scaling": "weak", "vs_baseline": None, "dtype": "u32",
                           "data": "synthetic (seeded counter-based generator)",
                           "config": {"workload": "ingest", "text_bytes": r["text_bytes"]}, "ingest": r}), flush=True)
+        return 0
+
+    if args.workload in SPLIT_WORKLOADS:
+        r = run_split(fa, args.workload, steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)
+        print(json.dumps({"metric": METRIC, "value": r["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+                          "data": "synthetic (seeded counter-based generator)",
+                          "config": {"workload": r["workload"]}, "roofline": r["roofline"],
+                          "cpu_baseline": r.get("cpu_baseline"), "split": r}), flush=True)
         return 0
 
     if args.workload == "serve-8sets":
@@ -631,6 +728,8 @@ def main() -> int:
         out["serving"] = run_serving(fa, n_requests=600)
         log("ingest leg (host memory / file) ...")
         out["ingest"] = run_ingest(fa, reps=2)
+        log("split filter leg (split-1m-2g-lines) ...")
+        out["split_method"] = run_split(fa, "split-1m-2g-lines", steps=10, cpu=not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, sbox, pb, po, text)
