@@ -311,6 +311,35 @@ int check_batch(ffbf_batch *b)
 
 }  // namespace
 
+namespace ffbf {
+int batch_reserve(ffbf_batch *b, uint32_t max_texts, uint64_t max_bytes)
+{
+    RET_IF(check_batch(b));
+    const uint32_t ci = b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u;
+    // chunks: every text adds at most two partial wave-iterations and one
+    // partial chunk beyond its share of the bytes
+    const uint64_t chunks = max_bytes / (16ull * kIterUnits * ci) + 3ull * max_texts + 16u;
+    const uint64_t cand = std::max<uint64_t>(1 << 16, max_bytes >> 10);
+    bool moved = false;
+    if (b->d_texts.cap < max_texts) {
+        RET_IF(b->d_texts.ensure(std::max<uint64_t>(max_texts, 1024), "batch text descriptors"));
+        moved = true;
+    }
+    if (b->d_chunks.cap < chunks) {
+        RET_IF(b->d_chunks.ensure(chunks, "batch chunks"));
+        moved = true;
+    }
+    if (b->cand.cap < cand) {
+        RET_IF(b->cand.ensure(cand, "batch candidates"));
+        RET_IF(b->cstage.ensure(b->n_cu, b->cand.cap));
+        RET_IF(refresh_filters(b));
+        moved = true;
+    }
+    if (moved) drop_graph(b);
+    return FFBF_OK;
+}
+}  // namespace ffbf
+
 extern "C" {
 
 int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch **out)

@@ -203,6 +203,11 @@ void ingest_release_staging(ffbf_ctx *c);
 void ingest_release(ffbf_ctx *c);
 // The stream a batch object queues its work on (ffbf_batch.cpp).
 hipStream_t batch_stream(ffbf_batch *b);
+// Sizes a batch's descriptor, chunk and candidate buffers for batches of up
+// to max_texts texts and max_bytes bytes, so that a stream of such batches
+// never reallocates (each move costs a device-wide hipMalloc and a graph
+// re-capture).
+int batch_reserve(ffbf_batch *b, uint32_t max_texts, uint64_t max_bytes);
 // Stage launchers shared with the batch path (ffbf_capi.cpp).
 int enqueue_pass(ffbf_ctx *c, bool fresh);
 int use_device(ffbf_ctx *c);

@@ -303,8 +303,8 @@ typedef struct ffbf_server ffbf_server;
 
 typedef struct ffbf_server_config {
     uint64_t hbm_budget_bytes;   /* device bytes of request data in flight (0 = 4 GiB) */
-    uint64_t max_batch_bytes;    /* text bytes per batch = per slot buffer (0 = budget / slots) */
-    uint32_t slots;              /* batches on the device at once (0 = 2) */
+    uint64_t max_batch_bytes;    /* text bytes per batch = per slot buffer (0 = min(budget / slots, 128 MiB)) */
+    uint32_t slots;              /* batches in flight at once (0 = 4) */
     uint32_t max_batch_requests; /* requests per batch (0 = 512) */
 } ffbf_server_config;
 
