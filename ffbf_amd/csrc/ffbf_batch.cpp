@@ -168,7 +168,6 @@ int refresh_filters(ffbf_batch *b)
         p.mask = c->mask;
         p.R = c->R.p;
         p.bitmap = c->bitmap.p;
-        p.surv_ids = c->surv_ids.p;
         p.counters = c->counters.p;
         p.table = c->table.p;
         p.slot_chain = c->table.p + c->table_size;

@@ -377,7 +377,6 @@ int enqueue_pass(ffbf_ctx *c, bool fresh)
     a.mask = c->mask;
     a.R = c->R.p;
     a.bitmap = c->bitmap.p;
-    a.surv_ids = c->surv_ids.p;
     a.counters = c->counters.p;
     a.table = c->table.p;
     a.slot_chain = c->table.p + c->table_size;
@@ -640,7 +639,6 @@ void ffbf_destroy(ffbf_ctx *c)
     c->off.release();
     c->pat_hash.release();
     c->bitmap.release();
-    c->surv_ids.release();
     c->table.release();
     c->chain_next.release();
     c->owned_text.release();
@@ -670,25 +668,23 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
     if (total && !bytes) return fail(FFBF_EINVAL, "null pattern bytes");
     uint64_t n_long = 0;
     for (uint64_t i = 0; i < n; i++) n_long += (offsets[i + 1] - offsets[i] >= c->params.window) ? 1u : 0u;
-    RET_IF(c->bytes.ensure(total, "pattern bytes"));
+    RET_IF(c->bytes.ensure(total + 64u, "pattern bytes"));  // padding: the build reads aligned 16-byte units
     RET_IF(c->off.ensure(n + 1, "pattern offsets"));
     RET_IF(c->pat_hash.ensure(n, "pattern hashes"));
     RET_IF(c->bitmap.ensure((n + 31) / 32, "survivor bitmap"));
-    RET_IF(c->surv_ids.ensure(n, "survivor ids"));
     RET_IF(c->chain_next.ensure(n, "survivor chains"));
     const uint32_t tl = std::max<uint32_t>(pow2_at_least(2 * n), 10);
     c->table_size = 1u << tl;
     RET_IF(c->table.ensure(2ull * c->table_size, "survivor index"));
     if (total) HIP_TRY(hipMemcpyAsync(c->bytes.p, bytes, total, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->off.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
-    RET_IF(clear_counters(c, ffbf::C_HASHED, 1));
     // prefix hash pairs only (F untouched until ffbf_build)
     if (c->split)
         HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->smap,
-                                         nullptr, c->pat_hash.p, c->counters.p, c->stream));
+                                         nullptr, c->pat_hash.p, c->stream));
     else
         HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->mask,
-                                   nullptr, c->pat_hash.p, c->counters.p, c->stream));
+                                   nullptr, c->pat_hash.p, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->n_pat = n;
     c->n_long = n_long;
@@ -725,13 +721,12 @@ int ffbf_build(ffbf_ctx *c)
     if (!c->patterns) return fail(FFBF_ESTATE, "build before patterns are loaded");
     RET_IF(record(c, EV_B0));
     HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
-    RET_IF(clear_counters(c, ffbf::C_HASHED, 1));
     if (c->split)
         HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
-                                         c->smap, c->F.p, c->pat_hash.p, c->counters.p, c->stream));
+                                         c->smap, c->F.p, c->pat_hash.p, c->stream));
     else
         HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
-                                   c->mask, c->F.p, c->pat_hash.p, c->counters.p, c->stream));
+                                   c->mask, c->F.p, c->pat_hash.p, c->stream));
     RET_IF(record(c, EV_B1));
     c->f_set_bits = -1;
     RET_IF(choose_mode(c));

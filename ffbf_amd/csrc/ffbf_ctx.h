@@ -144,7 +144,7 @@ struct ffbf_ctx {
     ffbf::DevBuf<uint8_t> bytes;
     ffbf::DevBuf<uint64_t> off;
     ffbf::DevBuf<uint2> pat_hash;
-    ffbf::DevBuf<uint32_t> bitmap, surv_ids, table, chain_next;
+    ffbf::DevBuf<uint32_t> bitmap, table, chain_next;
     uint64_t n_pat = 0;
     uint32_t table_size = 0;
 

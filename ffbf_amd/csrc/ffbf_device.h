@@ -27,7 +27,7 @@ static_assert(sizeof(MatchRec) == 16, "MatchRec is 16 bytes");
 // C_CAND: candidates found (beyond the list capacity when some were lost, so
 // that the caller grows the list and scans again); C_CVALID: leading entries
 // of the compacted list that hold candidates (what verification reads).
-enum Counter : int { C_CAND = 0, C_SURV = 1, C_MATCH = 2, C_HASHED = 3, C_SPILL = 4, C_CVALID = 5, C_COUNT = 8 };
+enum Counter : int { C_CAND = 0, C_SURV = 1, C_MATCH = 2, C_SPILL = 4, C_CVALID = 5, C_COUNT = 8 };
 
 // Candidate staging of a scan launch: wave w of the launch appends to its own
 // region of `region` entries at (wave0 + w) * region of the raw candidate
@@ -165,7 +165,6 @@ struct PassArgs {
     uint32_t W, k, mask;
     const uint32_t *R;
     uint32_t *bitmap;   // ceil(n/32) words, fully rewritten
-    uint32_t *surv_ids; // compacted survivor ids
     unsigned long long *counters;
     uint32_t *table;
     uint32_t *slot_chain;
@@ -186,13 +185,11 @@ struct ZeroSpan {
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
-                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, unsigned long long *counters,
-                        hipStream_t s);
+                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, hipStream_t s);
 // The split method's build: 64-bit prefix hashes from a 256-entry (lo, hi)
 // sbox, k bits placed by m (F null: hashes only).
 hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
-                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash,
-                              unsigned long long *counters, hipStream_t s);
+                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash, hipStream_t s);
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s);
 hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,

@@ -32,29 +32,45 @@ __global__ void k_copy(uint4 *dst, const uint4 *src, uint64_t n16)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Calls step(c) for the bytes c of bytes[a0 .. a0+W) in order, read with
+// aligned 16-byte loads (the pattern buffer is allocated with 64 bytes of
+// padding, so the last load stays inside it).
+template <class Step>
+__device__ __forceinline__ void prefix_bytes(const uint8_t *bytes, uint64_t a0, uint32_t W, Step step)
+{
+    const uint64_t end = a0 + W;
+    for (uint64_t q = a0 & ~15ull; q < end; q += 16u) {
+        const uint4 v = *(const uint4 *)(bytes + q);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
+            const uint64_t pos = q + (uint64_t)j;
+            if (pos >= a0 && pos < end) step((w >> (8 * (j & 3))) & 0xFFu);
+        }
+    }
+}
+
 // Filter build (grepSetup, greptest.cu:96-123): one lane per pattern hashes
 // its W-byte prefix and sets k forward bits; patterns shorter than W are
 // skipped (ffbf.cpp:271-275).  The hash pair is kept for the pattern pass and
 // verification; with F null only the hash pairs are computed.
 __global__ __launch_bounds__(256) void k_build(const uint8_t *bytes, const uint64_t *off, uint64_t n,
                                                const uint32_t *sbox, uint32_t W, uint32_t k, uint32_t mask,
-                                               uint32_t *F, uint2 *pat_hash, unsigned long long *counters)
+                                               uint32_t *F, uint2 *pat_hash)
 {
     __shared__ uint32_t sb[256];
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) sb[i] = sbox[i];
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint32_t hashed = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t a0 = off[i], a1 = off[i + 1];
         uint32_t h1 = 0, h2 = 0;
         if (a1 - a0 >= W) {
-            const uint8_t *p = bytes + a0;
-            for (uint32_t j = 0; j < W; j++) {
-                const uint32_t s = sb[p[j]];
+            prefix_bytes(bytes, a0, W, [&](uint32_t c) {
+                const uint32_t s = sb[c];
                 h1 = rotl(h1, 1) ^ s;
                 h2 = rotl(h2, 3) ^ s;
-            }
+            });
             if (F) {
                 for (int j = 0; j < 5; j++) {
                     if (j >= (int)k) break;
@@ -62,11 +78,9 @@ __global__ __launch_bounds__(256) void k_build(const uint8_t *bytes, const uint6
                     atomicOr(&F[idx >> 5], 1u << (idx & 31u));
                 }
             }
-            hashed++;
         }
         pat_hash[i] = make_uint2(h1, h2);
     }
-    if (hashed) atomicAdd(counters + C_HASHED, (unsigned long long)hashed);
 }
 
 // The split method's build (Filter::operator<<, ffbf.cpp:268-283): one lane
@@ -77,25 +91,23 @@ __global__ __launch_bounds__(256) void k_build(const uint8_t *bytes, const uint6
 // ffbf.cpp:196-203).  Hash halves are kept as (h1, h2) = (low, high).
 __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n,
                                                      const uint2 *sbox, uint32_t W, uint32_t k, SplitMap m,
-                                                     uint32_t *F, uint2 *pat_hash, unsigned long long *counters)
+                                                     uint32_t *F, uint2 *pat_hash)
 {
     __shared__ uint2 sb[256];
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) sb[i] = sbox[i];
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint32_t hashed = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t a0 = off[i], a1 = off[i + 1];
         uint32_t lo = 0, hi = 0;
         if (a1 - a0 >= W) {
-            const uint8_t *p = bytes + a0;
-            for (uint32_t j = 0; j < W; j++) {
-                const uint2 v = sb[p[j]];
+            prefix_bytes(bytes, a0, W, [&](uint32_t c) {
+                const uint2 v = sb[c];
                 uint32_t rl, rh;
                 rol64(lo, hi, 1u, rl, rh);
                 lo = rl ^ v.x;
                 hi = rh ^ v.y;
-            }
+            });
             if (F) {
                 for (int j = 0; j < kSplitMaxK; j++) {
                     if (j >= (int)k) break;
@@ -103,11 +115,9 @@ __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const
                     atomicOr(&F[idx >> 5], 1u << (idx & 31u));
                 }
             }
-            hashed++;
         }
         pat_hash[i] = make_uint2(lo, hi);
     }
-    if (hashed) atomicAdd(counters + C_HASHED, (unsigned long long)hashed);
 }
 
 // OR-fold of F onto fold_words words: fold bit x is set iff some F bit
@@ -130,7 +140,7 @@ __global__ void k_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint
 // The split method tests its k (<= 11) reverse functions, placed in the
 // cache or external part (checkInFilter_2, ffbf.cpp:241-266).
 template <bool SPLIT = false>
-__device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
+__device__ __forceinline__ uint32_t pass_wave(const PassArgs &a, uint64_t i0)
 {
     const int lane = lane_id();
     const uint64_t n_words = (a.n + 31u) / 32u;
@@ -148,26 +158,28 @@ __device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
     } else if (i < a.n) {
         ok = a.off[i + 1] - a.off[i] >= a.W;
         h = a.pat_hash[i];
-        uint32_t words[5], idx[5];
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-            idx[j] = rev_idx(j, h.x, h.y) & a.mask;
-            words[j] = (ok && j < (int)a.k) ? a.R[idx[j] >> 5] : 0xFFFFFFFFu;
+        // R is sparse: its first bit for every pattern, the others (loaded
+        // together) only where that one is set
+        if (ok) {
+            const uint32_t i0 = rev_idx(0, h.x, h.y) & a.mask;
+            ok = (a.R[i0 >> 5] >> (i0 & 31u)) & 1u;
         }
+        if (ok && a.k > 1u) {
+            uint32_t words[5], idx[5];
 #pragma unroll
-        for (int j = 0; j < 5; j++) ok = ok && ((words[j] >> (idx[j] & 31u)) & 1u);
+            for (int j = 1; j < 5; j++) {
+                idx[j] = rev_idx(j, h.x, h.y) & a.mask;
+                words[j] = j < (int)a.k ? a.R[idx[j] >> 5] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int j = 1; j < 5; j++) ok = ok && ((words[j] >> (idx[j] & 31u)) & 1u);
+        }
     }
     const uint64_t bal = __ballot(ok);
     const uint64_t w = i0 >> 5;
     if (lane == 0 && w < n_words) a.bitmap[w] = (uint32_t)bal;
     if (lane == 32 && w + 1 < n_words) a.bitmap[w + 1] = (uint32_t)(bal >> 32);
-    if (!bal) return;
-    const int leader = __builtin_ctzll(bal);
-    unsigned long long first = 0;
-    if (lane == leader) first = atomicAdd(a.counters + C_SURV, (unsigned long long)__builtin_popcountll(bal));
-    first = readlane64(first, leader);
-    if (!ok) return;
-    a.surv_ids[first + lanes_below(bal)] = (uint32_t)i;
+    if (!ok) return (uint32_t)__builtin_popcountll(bal);
     // index by key: the first id to claim a slot represents the key, the
     // rest chain from the slot
     uint32_t slot = key_slot(h.x, h.y) & a.table_mask;
@@ -181,6 +193,7 @@ __device__ __forceinline__ void pass_wave(const PassArgs &a, uint64_t i0)
         }
         slot = (slot + 1u) & a.table_mask;
     }
+    return (uint32_t)__builtin_popcountll(bal);
 }
 
 template <bool SPLIT>
@@ -188,7 +201,11 @@ __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
 {
     const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) pass_wave<SPLIT>(a, i0);
+    uint32_t n = 0;
+    for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) n += pass_wave<SPLIT>(a, i0);
+    // one count per wave: a counter shared by every wave-iteration of the
+    // launch would serialize thousands of atomics on one address
+    if (n && lane_id() == 0) atomicAdd(a.counters + C_SURV, (unsigned long long)n);
 }
 
 // The pattern passes of several filters in one launch: wave_start[f] is the
@@ -211,7 +228,8 @@ __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filt
         const PassArgs &a = filters[lo];
         const uint64_t i0 = (gw - wave_start[lo]) << 6;
         if (active[lo]) {
-            pass_wave<false>(a, i0);
+            const uint32_t n = pass_wave<false>(a, i0);
+            if (n && lane_id() == 0) atomicAdd(a.counters + C_SURV, (unsigned long long)n);
         } else {
             const uint64_t n_words = (a.n + 31u) / 32u, w = i0 >> 5;
             const int lane = lane_id();
@@ -282,26 +300,23 @@ hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s
 }
 
 hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
-                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, unsigned long long *counters,
-                        hipStream_t s)
+                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255u) / 256u;
     if (blocks > 8192u) blocks = 8192u;
-    hipLaunchKernelGGL(k_build, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, sbox, W, k, mask, F, pat_hash,
-                       counters);
+    hipLaunchKernelGGL(k_build, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, sbox, W, k, mask, F, pat_hash);
     return hipGetLastError();
 }
 
 hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
-                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash,
-                              unsigned long long *counters, hipStream_t s)
+                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255u) / 256u;
     if (blocks > 8192u) blocks = 8192u;
     hipLaunchKernelGGL(k_build_split, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, (const uint2 *)sbox64,
-                       W, k, m, F, pat_hash, counters);
+                       W, k, m, F, pat_hash);
     return hipGetLastError();
 }
 
