@@ -40,7 +40,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
     PendingCand pend;
     uint32_t loaded = 0xFFFFFFFFu;
     ScanArgs a{};
-    bool lds = false;
+    // the LDS-class launch only gets chunks of filters whose prefilter is
+    // held in LDS, the other launch only the rest: the probe path is fixed
+    // at compile time (one path's registers, not both)
+    constexpr bool lds = !GLOBAL_ONLY;
 
     for (;;) {
         __syncthreads();
@@ -52,18 +55,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         if (c.filter != loaded) {
             if (qn) {
                 wave_sync();
-                if (lds)
-                    drain<true, P>(q, qn, a, s_pre, pend);
-                else
-                    drain<false, 1>(q, qn, a, s_pre, pend);
+                drain<lds, lds ? P : 1>(q, qn, a, s_pre, pend);
                 wave_sync();
                 qn = 0;
             }
             if (loaded != 0xFFFFFFFFu) flush_pending(pend, a);
             a = b.filters[c.filter];
-            lds = !GLOBAL_ONLY && a.lds != 0u;
             __syncthreads();
-            if (lds) {
+            if constexpr (lds) {
                 const uint4 *src = (const uint4 *)a.pre;
                 uint4 *dst = (uint4 *)s_pre;
                 for (uint32_t i = threadIdx.x; i < a.pre_words / 4u; i += BLOCK) dst[i] = src[i];
@@ -97,18 +96,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
                 hash_unit_generic(base, lead, t.n, u, W, s_sbox, h1, h2);
             }
             const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
-            if (!GLOBAL_ONLY && lds)
-                test_and_queue<true, BLOCK, P>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
-            else
-                test_and_queue<false, BLOCK, 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
+            test_and_queue<lds, BLOCK, lds ? P : 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
         }
     }
     if (qn) {
         wave_sync();
-        if (lds)
-            drain<true, P>(q, qn, a, s_pre, pend);
-        else
-            drain<false, 1>(q, qn, a, s_pre, pend);
+        drain<lds, lds ? P : 1>(q, qn, a, s_pre, pend);
     }
     if (loaded != 0xFFFFFFFFu) finish_pending(pend, a);
 }
