@@ -115,6 +115,10 @@ int choose_mode_split(ffbf_ctx *c)
     if (c->mode_req == ffbf::MODE_FOLD && S == 0)
         return fail(FFBF_EINVAL, "split filter with s = 0 has no cache part to fold");
     const uint64_t cache_words = (uint64_t)m.cache / 32u;
+    // the scan copies a prefilter into LDS in 16-byte units: at least 4 words
+    const bool foldable = S > 0 && cache_words >= 4u;
+    if (c->mode_req == ffbf::MODE_FOLD && !foldable)
+        return fail(FFBF_EINVAL, "split filter: a cache part below 2^7 bits cannot be folded");
     const uint32_t max_words = ffbf::max_fold_words(c->device, true);
     uint32_t words = (uint32_t)std::min<uint64_t>(max_words, cache_words);
     if (c->fold_log2_req) {
@@ -132,7 +136,7 @@ int choose_mode_split(ffbf_ctx *c)
     uint32_t P = c->probes_req ? c->probes_req : (ff < 0.01 ? 1u : (ff < 0.2 ? 2u : 3u));
     P = std::max(1u, std::min(std::min(P, S), 3u));
     bool fold = false;
-    if (S > 0 && c->mode_req != ffbf::MODE_GLOBAL) {
+    if (foldable && c->mode_req != ffbf::MODE_GLOBAL) {
         if (c->mode_req == ffbf::MODE_FOLD) {
             fold = true;
         } else {
@@ -739,9 +743,18 @@ int ffbf_set_scan_mode(ffbf_ctx *c, int mode, uint32_t fold_log2_bits)
 {
     RET_IF(check_ctx(c));
     if (mode < -1 || mode > 3) return fail(FFBF_EINVAL, "mode must be -1..3");
+    const int old_mode = c->mode_req;
+    const uint32_t old_fold = c->fold_log2_req;
     c->mode_req = mode;
     c->fold_log2_req = fold_log2_bits;
-    if (c->built) RET_IF(choose_mode(c));
+    if (c->built) {
+        const int st = choose_mode(c);
+        if (st != FFBF_OK) {  // a rejected request leaves the previous one in force
+            c->mode_req = old_mode;
+            c->fold_log2_req = old_fold;
+            return st;
+        }
+    }
     return FFBF_OK;
 }
 
@@ -751,9 +764,17 @@ int ffbf_set_scan_tuning(ffbf_ctx *c, uint32_t probes, uint32_t block)
     if (probes > c->params.k) return fail(FFBF_EINVAL, "probes must be 0..k");
     if (block != 0 && block != 256 && block != 512 && block != 1024)
         return fail(FFBF_EINVAL, "block must be 0, 256, 512 or 1024");
+    const uint32_t old_probes = c->probes_req, old_block = c->block_req;
     c->probes_req = probes;
     c->block_req = block;
-    if (c->built) RET_IF(choose_mode(c));
+    if (c->built) {
+        const int st = choose_mode(c);
+        if (st != FFBF_OK) {
+            c->probes_req = old_probes;
+            c->block_req = old_block;
+            return st;
+        }
+    }
     return FFBF_OK;
 }
 

@@ -225,6 +225,13 @@ def test_split_rejected_where_unsupported():
         with pytest.raises(fa.FFBFError) as e:
             fa.Batch([m])
         assert e.value.status == fa.EINVAL
+    # a cache part of one word cannot be folded (auto picks F probed directly)
+    with make_split(19, 3, 3, 5, 10, sb, b, off) as m:
+        with pytest.raises(fa.FFBFError) as e:
+            m.set_scan_mode(fa.MODE_FOLD)
+        assert e.value.status == fa.EINVAL
+        m.set_scan_mode(fa.MODE_AUTO)
+        assert m.timings()["scan_mode"] == fa.MODE_GLOBAL
     for bad in [(19, 12, 2, 24, 28), (19, 5, 6, 24, 28), (19, 5, 2, 4, 28), (19, 5, 2, 24, 32), (0, 5, 2, 24, 28)]:
         with pytest.raises(fa.FFBFError) as e:
             fa.Matcher.split(*bad, sb)
