@@ -123,3 +123,95 @@ def test_split_method_random(case):
         m.set_scan_mode(mode, fold)
         m.set_scan_tuning(probes, 0)
         check(run_gpu(m, text, shift), r, m.filter(fa.FILTER_FORWARD))
+
+
+def _batch_case(rng):
+    """Random resident filters (mode, size, k fixed per batch) and texts."""
+    W = int(rng.choice([12, 19, 24, 32, 40]))
+    k = int(rng.integers(1, 6))
+    sbox = fa.sbox_from_seed(int(rng.integers(1, 1 << 30)))
+    nf = int(rng.integers(1, 9))
+    filters = []
+    for f in range(nf):
+        log2 = int(rng.integers(10, 24))
+        mode = int(rng.choice([fa.MODE_AUTO, fa.MODE_GLOBAL, fa.MODE_FOLD, fa.MODE_LDS, fa.MODE_GFOLD]))
+        if mode == fa.MODE_LDS and log2 > 20:
+            mode = fa.MODE_FOLD
+        fold = {fa.MODE_FOLD: int(rng.integers(7, min(log2, 20) + 1)),
+                fa.MODE_GFOLD: int(rng.integers(10, log2 + 1))}.get(mode, 0)
+        alpha = fa.ALPHA_ASCII if rng.random() < 0.5 else fa.ALPHA_DNA_GC40
+        b, off = fa.gen_patterns(alpha, int(rng.integers(1, 1 << 30)), int(rng.integers(0, 4000)),
+                                 max(1, W - 3), W + 40)
+        m = fa.Matcher(W, k, log2, sbox)
+        m.set_patterns(b, off)
+        m.build()
+        m.set_scan_mode(mode, fold)
+        filters.append((m, b, off, log2, alpha))
+    n_texts = int(rng.integers(1, 120))
+    lens = rng.integers(0, int(rng.choice([100, 5000, 300_000])), size=n_texts)
+    route = rng.integers(0, nf, size=n_texts).astype(np.uint32)
+    texts = []
+    for i in range(n_texts):
+        _, b, off, _, alpha = filters[int(route[i])]
+        t = fa.gen_bytes(alpha, int(rng.integers(1, 1 << 30)), int(lens[i]))
+        if t.size > 200 and off.size > 1:
+            fa.plant(int(rng.integers(1, 1 << 30)), 1 + t.size // 10000, t, b, off)
+        texts.append(t)
+    return W, k, sbox, filters, texts, route
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_batch_random(case):
+    rng = np.random.default_rng(9800 + case)
+    W, k, sbox, filters, texts, route = _batch_case(rng)
+    tuning = dict(chunk_iters=int(rng.choice([0, 1, 5, 32, 300])), probes=int(rng.integers(0, min(3, k) + 1)),
+                  block=int(rng.choice([0, 512, 1024])), use_graph=bool(rng.random() < 0.7))
+    say(f"batch case {case}: W={W} k={k} filters={len(filters)} texts={len(texts)} "
+        f"bytes={sum(t.size for t in texts)} tuning={tuning}")
+    rows = []
+    for i, t in enumerate(texts):
+        _, b, off, log2, _ = filters[int(route[i])]
+        r = Oracle(W, k, log2, sbox).run(t, b, off)
+        rows += [(i, int(p), int(q)) for p, q in zip(r["ids"], r["offs"])]
+    rows.sort()
+    lens = np.array([t.size for t in texts], dtype=np.uint64)
+    offs = np.zeros(len(texts) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts) if offs[-1] else np.zeros(1, np.uint8))
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_tuning(**tuning)
+        bt.set_texts(dev.ptr, offs, route)
+        n = bt.run()
+        m = bt.matches()
+        got = list(zip(m["text_id"].tolist(), m["pattern_id"].tolist(), m["offset"].tolist()))
+        assert n == len(rows) and got == rows
+    dev.free()
+    for f in filters:
+        f[0].close()
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_host_ingest_random(case):
+    rng = np.random.default_rng(9900 + case)
+    W = int(rng.choice([7, 19, 32]))
+    sbox = fa.sbox_from_seed(int(rng.integers(1, 1 << 30)))
+    b, off = fa.gen_patterns(fa.ALPHA_ASCII, int(rng.integers(1, 1 << 30)), 2000, W, W + 30)
+    chunk = int(rng.choice([1, 2, 3])) << 20
+    depth, threads = int(rng.integers(2, 5)), int(rng.integers(1, 9))
+    n = int(rng.choice([chunk - 1, chunk, chunk + W // 2, 3 * chunk + int(rng.integers(0, 1 << 20)),
+                        int(rng.integers(1, 6 << 20))]))
+    text = fa.gen_bytes(fa.ALPHA_ASCII, int(rng.integers(1, 1 << 30)), n)
+    fa.plant(int(rng.integers(1, 1 << 30)), 1 + n // 50000, text, b, off)
+    say(f"ingest case {case}: W={W} chunk={chunk} depth={depth} threads={threads} n={n}")
+    r = Oracle(W, 4, 20, sbox).run(text, b, off, threads=8)
+    with fa.Matcher(W, 4, 20, sbox) as m:
+        m.set_patterns(b, off)
+        m.build()
+        m.set_ingest(chunk, depth, threads)
+        m.scan(text)
+        assert np.array_equal(m.candidates(), r["cand"])
+        assert np.array_equal(m.filter(fa.FILTER_REVERSE), r["R"])
+        m.pattern_pass()
+        m.verify()
+        got = m.matches()
+        assert np.array_equal(got["pattern_id"], r["ids"]) and np.array_equal(got["offset"], r["offs"])
