@@ -116,15 +116,18 @@ int refresh_filters(ffbf_batch *b)
         }
     }
     b->probes_eff = std::max(1u, std::min({p, 3u, b->k}));
+    // a split filter's fold covers only its s cache-part functions: the
+    // launch-wide up-front probes may not exceed any such s
+    for (uint32_t f = 0; f < nf; f++) {
+        const ffbf_ctx *c = b->filters[f];
+        if (c->split && c->mode == MODE_FOLD) b->probes_eff = std::max(1u, std::min(b->probes_eff, c->smap.s));
+    }
     for (uint32_t f = 0; f < nf; f++) {
         const ffbf_ctx *c = b->filters[f];
         ScanArgs &a = sa[f];
         a = ScanArgs{};
         const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
-        a.F = c->F.p;
-        a.R = c->R.p;
-        a.pre = c->mode == MODE_FOLD ? c->fold.p : c->mode == MODE_GFOLD ? c->gfold.p : c->F.p;
-        a.sbox = c->sbox.p;
+        filter_scan_args(c, a);
         // region slots: LDS-class launch [0, half), global-class launch [half, all)
         const uint32_t half = b->cstage.n_regions / 2u;
         if (lds)
@@ -132,15 +135,7 @@ int refresh_filters(ffbf_batch *b)
         else
             b->cstage.fill(a, half, b->cstage.n_regions);
         a.counters = b->counters.p;
-        a.W = c->params.window;
-        a.k = c->params.k;
-        a.mask = c->mask;
-        a.pre_words = c->pre_words;
-        a.pre_mask = c->mode == MODE_GLOBAL ? c->mask : (uint32_t)((uint64_t)c->pre_words * 32u - 1u);
-        a.pre_log2w = pow2_at_least(c->pre_words);
         a.P = lds ? b->probes_eff : 1u;
-        a.pre_exact = (c->mode == MODE_GLOBAL || c->mode == MODE_LDS) ? 1u : 0u;
-        a.lds = lds ? 1u : 0u;
         VerifyArgs &v = va[f];
         v = VerifyArgs{};
         v.bytes = c->bytes.p;
@@ -173,6 +168,8 @@ int refresh_filters(ffbf_batch *b)
         p.slot_chain = c->table.p + c->table_size;
         p.chain_next = c->chain_next.p;
         p.table_mask = c->table_size - 1u;
+        p.method = c->split ? METHOD_SPLIT : METHOD_PAIR;
+        p.split = c->smap;
         ws[f + 1] = ws[f] + (c->n_pat + 63u) / 64u;
     }
     b->pass_waves = ws[nf];
@@ -219,6 +216,7 @@ int enqueue_run(ffbf_batch *b, bool timed)
     ba.filters = b->d_scan.p;
     ba.sbox = b->filters[0]->sbox.p;
     ba.W = b->W;
+    ba.method = b->filters[0]->split ? METHOD_SPLIT : METHOD_PAIR;
     if (b->has_lds) {
         ba.ctl = b->ctl.p;
         HIP_TRY(launch_scan_batch(ba, b->tuning.block, b->probes_eff, b->pre_words_max, b->n_cu, b->stream));
@@ -232,7 +230,7 @@ int enqueue_run(ffbf_batch *b, bool timed)
                                 b->cstage.n_regions, b->cstage.region, b->cand.p, b->cand.cap, b->counters.p, b->n_cu,
                                 b->stream));
     HIP_TRY(launch_pattern_pass_multi(b->d_pass.p, b->d_wave_start.p, (uint32_t)b->filters.size(), b->d_active.p,
-                                      b->pass_waves, b->n_cu, b->stream));
+                                      b->pass_waves, b->n_cu, b->stream, b->filters[0]->split));
     BatchVerifyArgs va{};
     va.texts = b->texts;
     va.tdesc = b->d_texts.p;
@@ -350,11 +348,15 @@ int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch *
     for (uint32_t i = 0; i < n_filters; i++) {
         const ffbf_ctx *c = filters[i];
         if (!c) return fail(FFBF_EINVAL, "null filter");
-        if (c->split) return fail(FFBF_EINVAL, "batches take contexts of ffbf_create (not the split filter)");
         if (!c->built || !c->patterns) return fail(FFBF_ESTATE, "every filter must have patterns and a built filter");
         if (c->device != f0->device || c->params.window != f0->params.window || c->params.k != f0->params.k ||
             std::memcmp(c->params.sbox, f0->params.sbox, sizeof f0->params.sbox) != 0)
             return fail(FFBF_EINVAL, "filters of a batch need the same device, window, k and sbox");
+        // one hash method per batch; split filters also share their 64-bit
+        // table (their s and part sizes may differ)
+        if (c->split != f0->split ||
+            (c->split && std::memcmp(c->split_params.sbox, f0->split_params.sbox, sizeof f0->split_params.sbox) != 0))
+            return fail(FFBF_EINVAL, "filters of a batch need the same hash method (and split filters the same sbox)");
     }
     ffbf_batch *b = new (std::nothrow) ffbf_batch();
     if (!b) return fail(FFBF_ENOMEM, "batch allocation");

@@ -20,16 +20,19 @@ namespace {
 // GLOBAL_ONLY: every filter of the launch keeps its prefilter in global
 // memory (F, or an L2-sized fold); no LDS is reserved for one, so small
 // blocks fit many waves per CU to hide the probes' round trips.
-template <int WT, int BLOCK, int P, bool GLOBAL_ONLY>
+// SPLIT: filters of the CPU binary's method (64-bit hash from a (lo, hi)
+// sbox, windows holding a newline never tested, cache-part fold).
+template <int WT, int BLOCK, int P, bool GLOBAL_ONLY, bool SPLIT = false>
 __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
 {
     using QG = QueueGeom<BLOCK>;
+    constexpr uint32_t kSboxWords = SPLIT ? 512u : 256u;
     extern __shared__ __align__(16) uint32_t s_pre[];
-    __shared__ uint32_t s_sbox[256];
+    __shared__ __align__(8) uint32_t s_sbox[kSboxWords];
     __shared__ uint4 s_q[BLOCK / 64][QG::cap];
     __shared__ uint32_t s_next;
 
-    for (uint32_t i = threadIdx.x; i < 256u; i += BLOCK) s_sbox[i] = b.sbox[i];
+    for (uint32_t i = threadIdx.x; i < kSboxWords; i += BLOCK) s_sbox[i] = b.sbox[i];
 
     const uint32_t W = WT ? (uint32_t)WT : b.W;
     const int wv = (int)(threadIdx.x >> 6);
@@ -55,7 +58,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         if (c.filter != loaded) {
             if (qn) {
                 wave_sync();
-                drain<lds, lds ? P : 1>(q, qn, a, s_pre, pend);
+                drain<lds, lds ? P : 1, SPLIT>(q, qn, a, s_pre, pend);
                 wave_sync();
                 qn = 0;
             }
@@ -86,22 +89,29 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         for (; it < it_end; it += BLOCK / 64) {
             const uint64_t u = it * kIterUnits + (uint64_t)lane_id();
             uint32_t h1[kUnit], h2[kUnit];
+            uint32_t line_ok = 0xFFFFu;
             if constexpr (WT > 0) {
                 uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
                 if (it + BLOCK / 64 < it_end)
                     nxt = load_chunk(base, lead, t.n, (it + BLOCK / 64) * kIterUnits + (uint64_t)lane_id());
-                hash_unit<(WT > 0 ? WT : 1)>(cur, s_sbox, h1, h2);
+                if constexpr (SPLIT)
+                    line_ok = hash_unit64<(WT > 0 ? WT : 1)>(cur, (const uint2 *)s_sbox, h1, h2);
+                else
+                    hash_unit<(WT > 0 ? WT : 1)>(cur, s_sbox, h1, h2);
                 cur = nxt;
             } else {
-                hash_unit_generic(base, lead, t.n, u, W, s_sbox, h1, h2);
+                if constexpr (SPLIT)
+                    line_ok = hash_unit_generic64(base, lead, t.n, u, W, (const uint2 *)s_sbox, h1, h2);
+                else
+                    hash_unit_generic(base, lead, t.n, u, W, s_sbox, h1, h2);
             }
             const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
-            test_and_queue<lds, BLOCK, lds ? P : 1>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
+            test_and_queue<lds, BLOCK, lds ? P : 1, SPLIT>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend, line_ok);
         }
     }
     if (qn) {
         wave_sync();
-        drain<lds, lds ? P : 1>(q, qn, a, s_pre, pend);
+        drain<lds, lds ? P : 1, SPLIT>(q, qn, a, s_pre, pend);
     }
     if (loaded != 0xFFFFFFFFu) finish_pending(pend, a);
 }
@@ -134,22 +144,22 @@ __global__ __launch_bounds__(256) void k_verify_batch(BatchVerifyArgs v)
     }
 }
 
-template <int WT, int BLOCK, int P, bool GLOBAL_ONLY = false>
+template <int WT, int BLOCK, int P, bool GLOBAL_ONLY = false, bool SPLIT = false>
 hipError_t launch_batch_t(const BatchArgs &a, uint32_t pre_words_max, int n_cu, hipStream_t s)
 {
     const size_t dyn = GLOBAL_ONLY ? 0u : (size_t)pre_words_max * 4u;
     if (dyn > 0) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
     }
     int per_cu = 0;
-    hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY>, BLOCK, dyn);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT>,
+                                                                BLOCK, dyn);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY>), dim3((uint32_t)(n_cu * per_cu)), dim3(BLOCK), dyn, s,
-                       a);
+    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT>), dim3((uint32_t)(n_cu * per_cu)), dim3(BLOCK),
+                       dyn, s, a);
     return hipGetLastError();
 }
 
@@ -169,11 +179,24 @@ hipError_t launch_batch_w(const BatchArgs &a, uint32_t block, uint32_t probes, u
     return launch_batch_p<WT, 1024>(a, probes, pre_words_max, n_cu, s);
 }
 
+// Split filters: 1024-thread blocks for the LDS class (any requested block
+// size), the window length of the CPU binary (19) or a run-time one.
+template <int WT>
+hipError_t launch_batch_split(const BatchArgs &a, uint32_t probes, uint32_t pre_words_max, int n_cu, hipStream_t s)
+{
+    if (probes <= 1u) return launch_batch_t<WT, 1024, 1, false, true>(a, pre_words_max, n_cu, s);
+    if (probes == 2u) return launch_batch_t<WT, 1024, 2, false, true>(a, pre_words_max, n_cu, s);
+    return launch_batch_t<WT, 1024, 3, false, true>(a, pre_words_max, n_cu, s);
+}
+
 }  // namespace
 
 hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
                              hipStream_t s)
 {
+    if (a.method == METHOD_SPLIT)
+        return a.W == 19 ? launch_batch_split<19>(a, probes, pre_words_max, n_cu, s)
+                         : launch_batch_split<0>(a, probes, pre_words_max, n_cu, s);
     switch (a.W) {
     case 19: return launch_batch_w<19>(a, block, probes, pre_words_max, n_cu, s);
     case 24: return launch_batch_w<24>(a, block, probes, pre_words_max, n_cu, s);
@@ -184,6 +207,9 @@ hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes
 
 hipError_t launch_scan_batch_global(const BatchArgs &a, int n_cu, hipStream_t s)
 {
+    if (a.method == METHOD_SPLIT)
+        return a.W == 19 ? launch_batch_t<19, 256, 1, true, true>(a, 0u, n_cu, s)
+                         : launch_batch_t<0, 256, 1, true, true>(a, 0u, n_cu, s);
     switch (a.W) {
     case 19: return launch_batch_t<19, 256, 1, true>(a, 0u, n_cu, s);
     case 24: return launch_batch_t<24, 256, 1, true>(a, 0u, n_cu, s);

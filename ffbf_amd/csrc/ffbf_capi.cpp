@@ -309,20 +309,11 @@ int enqueue_scan_begin(ffbf_ctx *c)
     return FFBF_OK;
 }
 
-// Scans the windows of text[0..n) (device memory), recording their offsets
-// plus base: a text can be scanned piece by piece, each piece starting at the
-// first window the previous one did not cover and carrying its W-1 byte tail.
-int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end)
+void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
 {
-    ffbf::ScanArgs a{};
-    a.text = text;
-    a.n = n;
-    a.off_base = base;
     a.F = c->F.p;
     a.R = c->R.p;
     a.sbox = c->sbox.p;
-    c->cstage.fill(a, 0u, c->cstage.n_regions);
-    a.counters = c->counters.p;
     a.W = c->params.window;
     a.k = c->params.k;
     a.mask = c->mask;
@@ -336,6 +327,8 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
     a.pre_log2w = pow2_at_least(c->pre_words);
     a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
+    a.lds = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
+    a.method = ffbf::METHOD_PAIR;
     if (c->split) {
         a.method = ffbf::METHOD_SPLIT;
         a.split = c->smap;
@@ -347,6 +340,20 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
             a.pre_mask = c->smap.s > 0 ? c->smap.cmask : c->smap.emask;
         }
     }
+}
+
+// Scans the windows of text[0..n) (device memory), recording their offsets
+// plus base: a text can be scanned piece by piece, each piece starting at the
+// first window the previous one did not cover and carrying its W-1 byte tail.
+int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end)
+{
+    ffbf::ScanArgs a{};
+    filter_scan_args(c, a);
+    a.text = text;
+    a.n = n;
+    a.off_base = base;
+    c->cstage.fill(a, 0u, c->cstage.n_regions);
+    a.counters = c->counters.p;
     HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
     if (mark_end) RET_IF(record(c, EV_S2));  // the scan kernel alone ends here
     HIP_TRY(ffbf::launch_cand_compact(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,

@@ -210,5 +210,9 @@ hipStream_t batch_stream(ffbf_batch *b);
 int batch_reserve(ffbf_batch *b, uint32_t max_texts, uint64_t max_bytes);
 // Stage launchers shared with the batch path (ffbf_capi.cpp).
 int enqueue_pass(ffbf_ctx *c, bool fresh);
+// The filter fields of a scan launch over context c (both hash methods):
+// filters, sbox, prefilter and its geometry, probes; not the text, staging
+// or counters.
+void filter_scan_args(const ffbf_ctx *c, ScanArgs &a);
 int use_device(ffbf_ctx *c);
 }  // namespace ffbf

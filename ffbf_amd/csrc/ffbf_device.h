@@ -124,9 +124,10 @@ struct BatchArgs {
     const BatchText *tdesc;
     const BatchChunk *chunks;
     const ScanArgs *filters;  // per filter; cand/counters are the batch's
-    const uint32_t *sbox;
+    const uint32_t *sbox;     // split method: 256 (lo, hi) pairs
     unsigned long long *ctl;  // [0] next chunk, [1] number of chunks, [2] index of the first, [3] number of texts
     uint32_t W;
+    uint32_t method;          // HashMethod of every filter of the batch
 };
 
 struct VerifyArgs {
@@ -198,7 +199,8 @@ hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
 // Pattern passes of nf filters in one launch (wave_start[nf+1], device
 // arrays); filters whose active flag is 0 get an empty survivor set.
 hipError_t launch_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start, uint32_t nf,
-                                     const uint32_t *active, uint64_t total_waves, int n_cu, hipStream_t s);
+                                     const uint32_t *active, uint64_t total_waves, int n_cu, hipStream_t s,
+                                     bool split = false);
 // Clears every span of the list in one launch.
 hipError_t launch_zero_spans(const ZeroSpan *spans, uint32_t n, int n_cu, hipStream_t s);
 // Appends the regions and spill of the raw candidate buffer to the list

@@ -212,6 +212,7 @@ __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
 // first global 64-id wave of filter f (prefix over ceil(n_f/64)).  A filter
 // whose `active` flag is clear (no text of the batch routed to it) has no
 // survivors: its bitmap words are cleared and nothing is indexed.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start,
                                                             uint32_t nf, const uint32_t *active)
 {
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filt
         const PassArgs &a = filters[lo];
         const uint64_t i0 = (gw - wave_start[lo]) << 6;
         if (active[lo]) {
-            const uint32_t n = pass_wave<false>(a, i0);
+            const uint32_t n = pass_wave<SPLIT>(a, i0);
             if (n && lane_id() == 0) atomicAdd(a.counters + C_SURV, (unsigned long long)n);
         } else {
             const uint64_t n_words = (a.n + 31u) / 32u, w = i0 >> 5;
@@ -342,13 +343,18 @@ hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s)
 }
 
 hipError_t launch_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start, uint32_t nf,
-                                     const uint32_t *active, uint64_t total_waves, int n_cu, hipStream_t s)
+                                     const uint32_t *active, uint64_t total_waves, int n_cu, hipStream_t s, bool split)
 {
     if (total_waves == 0) return hipSuccess;
     uint64_t blocks = (total_waves + 3u) / 4u;
     const uint64_t cap = (uint64_t)n_cu * 8u;
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(k_pattern_pass_multi, dim3((uint32_t)blocks), dim3(256), 0, s, filters, wave_start, nf, active);
+    if (split)
+        hipLaunchKernelGGL(k_pattern_pass_multi<true>, dim3((uint32_t)blocks), dim3(256), 0, s, filters, wave_start, nf,
+                           active);
+    else
+        hipLaunchKernelGGL(k_pattern_pass_multi<false>, dim3((uint32_t)blocks), dim3(256), 0, s, filters, wave_start,
+                           nf, active);
     return hipGetLastError();
 }
 

@@ -115,12 +115,13 @@ const char *ffbf_version(void);
 int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out);
 
 /* Creates a context of the CPU binary's split filter (Filter<H, LEN>
- * constructor, ffbf.cpp:88-108).  Every single-text entry point below works
- * on it unchanged (patterns, build, scan, pattern pass, verification,
- * outputs, persistence over its 2^c + 2^e bits); batches and servers take
- * contexts of ffbf_create only.  Scan modes: -1 automatic, 0 (F probed
- * directly; the cache part first) or 1 (an LDS fold of the cache part,
- * needs s >= 1). */
+ * constructor, ffbf.cpp:88-108).  Every entry point below works on it
+ * unchanged (patterns, build, scan, pattern pass, verification, outputs,
+ * persistence over its 2^c + 2^e bits, batches and servers); the filters of
+ * one batch or server share one hash method, and split filters there share
+ * W, k and the table.  Scan modes: -1 automatic, 0 (F probed directly; the
+ * cache part first) or 1 (an LDS fold of the cache part, of at most 2^19
+ * bits; needs s >= 1 and a cache part of at least 2^7 bits). */
 int ffbf_create_split(const ffbf_split_params *params, int device, ffbf_ctx **out);
 void ffbf_destroy(ffbf_ctx *ctx);
 

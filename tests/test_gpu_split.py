@@ -213,7 +213,7 @@ def test_split_ingest_outputs_and_persistence(tmp_path):
 
 
 def test_split_rejected_where_unsupported():
-    say("split: batch and mode errors")
+    say("split: mode and mixing errors")
     sb = sbox64(16)
     b, off = fa.gen_patterns(fa.ALPHA_ASCII, 17, 100, 19, 30)
     with make_split(19, 5, 2, 16, 18, sb, b, off) as m:
@@ -222,9 +222,13 @@ def test_split_rejected_where_unsupported():
                 m.set_scan_mode(mode)
             assert e.value.status == fa.EINVAL
         m.set_scan_mode(fa.MODE_AUTO)
-        with pytest.raises(fa.FFBFError) as e:
-            fa.Batch([m])
-        assert e.value.status == fa.EINVAL
+        # one hash method per batch
+        with fa.Matcher(19, 5, 20, fa.sbox_from_seed(3)) as pair:
+            pair.set_patterns(b, off)
+            pair.build()
+            with pytest.raises(fa.FFBFError) as e:
+                fa.Batch([m, pair])
+            assert e.value.status == fa.EINVAL
     # a cache part of one word cannot be folded (auto picks F probed directly)
     with make_split(19, 3, 3, 5, 10, sb, b, off) as m:
         with pytest.raises(fa.FFBFError) as e:
@@ -236,3 +240,60 @@ def test_split_rejected_where_unsupported():
         with pytest.raises(fa.FFBFError) as e:
             fa.Matcher.split(*bad, sb)
         assert e.value.status == fa.EINVAL
+
+
+def split_batch_filters(W, k, sb, shapes, seed):
+    out = []
+    for i, (s, c, e, mode, n_pat) in enumerate(shapes):
+        b, off = fa.gen_patterns(fa.ALPHA_ASCII, seed + i, n_pat, W, W + 40)
+        m = make_split(W, k, s, c, e, sb, b, off)
+        m.set_scan_mode(mode)
+        out.append((m, b, off, (s, c, e)))
+    return out
+
+
+@pytest.mark.parametrize("W", [19, 23])
+def test_split_batch_and_server(W):
+    say(f"split batch and server W={W}")
+    sb = sbox64(40 + W)
+    k = 5
+    shapes = [(2, 20, 24, fa.MODE_FOLD, 3000), (2, 24, 28, fa.MODE_GLOBAL, 20000), (1, 16, 22, fa.MODE_AUTO, 500),
+              (0, 10, 18, fa.MODE_GLOBAL, 2000), (3, 18, 20, fa.MODE_FOLD, 8000)]
+    filters = split_batch_filters(W, k, sb, shapes, 700)
+    rng = np.random.default_rng(W)
+    n_texts = 70
+    route = rng.integers(0, len(filters), size=n_texts).astype(np.uint32)
+    texts = []
+    for i in range(n_texts):
+        _, b, off, _ = filters[int(route[i])]
+        n = int(rng.integers(0, 150_000)) if i > 3 else [0, 3, W - 1, W][i]
+        texts.append(lines_text(900 + i, n, b, off, line_mean=int(rng.integers(20, 200)), plants=1 + n // 20000))
+    rows = []
+    for i, t in enumerate(texts):
+        _, b, off, (s, c, e) = filters[int(route[i])]
+        r = SplitOracle(W, k, s, c, e, sb).run(t, b, off)
+        rows += [(i, int(p), int(q)) for p, q in zip(r["ids"], r["offs"])]
+    rows.sort()
+    assert len(rows) > 30
+    offs = np.zeros(n_texts + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([t.size for t in texts])
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    with fa.Batch([f[0] for f in filters]) as bt:
+        for tuning in (dict(), dict(chunk_iters=3, probes=3, use_graph=False), dict(chunk_iters=200, probes=1)):
+            bt.set_tuning(**tuning)
+            bt.set_texts(dev.ptr, offs, route)
+            assert bt.run() == len(rows)
+            m = bt.matches()
+            assert list(zip(m["text_id"].tolist(), m["pattern_id"].tolist(), m["offset"].tolist())) == rows
+        assert bt.run_per_text() == len(rows)
+    dev.free()
+    # the serving loop over the same split filters: every request's matches
+    # equal its rows above
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=3) as srv:
+        ids = [srv.submit(int(route[i]), texts[i]) for i in range(n_texts)]
+        for i, rid in enumerate(ids):
+            got, _ = srv.wait(rid)
+            want = [(p, q) for (t, p, q) in rows if t == i]
+            assert list(zip(got["pattern_id"].tolist(), got["offset"].tolist())) == want, i
+    for f in filters:
+        f[0].close()
