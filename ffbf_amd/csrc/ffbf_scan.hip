@@ -178,8 +178,8 @@ uint32_t max_fold_words(int device, bool split)
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds <= 0)
         lds = 64 * 1024;
     // static LDS of the 512-thread scan: sbox 1 KiB + 8 queues of 3.75 KiB;
-    // the split method's 1024-thread scans: sbox 2 KiB + 16 queues of 1.875
-    // KiB, plus the batched scan's chunk word (with room for alignment)
+    // the split method's 1024-thread scans: sbox 2 KiB + 16 queues of 119
+    // entries, plus the batched scan's chunk word (with room for alignment)
     const int static_lds = split ? 2048 + 16 * QueueGeom<1024>::cap * 16 + 64 : 1024 + 8 * QueueGeom<512>::cap * 16;
     uint32_t words = 1u << 15;  // 128 KiB
     while (words > 256u && (int)(words * 4u) + static_lds > lds) words >>= 1;

@@ -23,10 +23,13 @@ struct QueueGeom {
     static constexpr int cap = 128;
     static constexpr int drain_at = 64;
 };
+// 1024-thread blocks beside a 128 KiB prefilter: 16 queues, sized so that
+// the split method's 2 KiB sbox and the batched scan's chunk word still fit
+// in 160 KiB
 template <>
 struct QueueGeom<1024> {
-    static constexpr int cap = 120;
-    static constexpr int drain_at = 57;
+    static constexpr int cap = 119;
+    static constexpr int drain_at = 56;
 };
 // 512-thread blocks: what LDS is left beside a 128 KiB prefilter, for filters
 // that pass many windows

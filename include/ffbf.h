@@ -120,7 +120,7 @@ int ffbf_create(const ffbf_params *params, int device, ffbf_ctx **out);
  * persistence over its 2^c + 2^e bits, batches and servers); the filters of
  * one batch or server share one hash method, and split filters there share
  * W, k and the table.  Scan modes: -1 automatic, 0 (F probed directly; the
- * cache part first) or 1 (an LDS fold of the cache part, of at most 2^19
+ * cache part first) or 1 (an LDS fold of the cache part, of at most 2^20
  * bits; needs s >= 1 and a cache part of at least 2^7 bits). */
 int ffbf_create_split(const ffbf_split_params *params, int device, ffbf_ctx **out);
 void ffbf_destroy(ffbf_ctx *ctx);
