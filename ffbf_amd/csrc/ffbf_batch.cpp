@@ -280,7 +280,9 @@ int run_once(ffbf_batch *b)
 int sort_on_device(ffbf_batch *b)
 {
     b->sorted_on_device = false;
-    if (b->n_match < 2) return FFBF_OK;
+    // a few thousand records sort faster on the host at copy-out than the
+    // device radix sort's launches take
+    if (b->n_match < kDeviceSortMin) return FFBF_OK;
     uint64_t max_pat = 1;
     for (const ffbf_ctx *c : b->filters) max_pat = std::max<uint64_t>(max_pat, c->n_pat);
     const uint32_t bo = bits_for(b->max_text), bi = bits_for(max_pat), bt = bits_for(b->n_texts);

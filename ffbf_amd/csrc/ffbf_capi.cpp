@@ -465,7 +465,9 @@ int do_scan(ffbf_ctx *c)
 int sort_on_device(ffbf_ctx *c)
 {
     c->sorted_on_device = false;
-    if (c->n_match < 2 || !ffbf::sort_packable(c->n_pat, c->text_n)) return FFBF_OK;
+    // a few thousand records sort faster on the host at copy-out than the
+    // device radix sort's launches take
+    if (c->n_match < ffbf::kDeviceSortMin || !ffbf::sort_packable(c->n_pat, c->text_n)) return FFBF_OK;
     const int end_bit = ffbf::sort_end_bit(c->n_pat);
     size_t tb = 0;
     HIP_TRY(ffbf::sort_temp_bytes(c->n_match, end_bit, &tb));

@@ -219,6 +219,9 @@ hipError_t launch_verify_batch(const BatchVerifyArgs &a, int n_cu, hipStream_t s
 hipError_t sort_batch_matches(BatchMatchRec *m, uint64_t n, uint64_t *keys_a, uint64_t *keys_b, void *temp,
                               size_t temp_bytes, int id_shift, int text_shift, int end_bit, hipStream_t s);
 
+// Match lists shorter than this are sorted on the host at copy-out.
+constexpr uint64_t kDeviceSortMin = 4096;
+
 // Device sort of matches by (id, offset) through packed 64-bit keys.
 bool sort_packable(uint64_t n_patterns, uint64_t text_bytes);
 int sort_end_bit(uint64_t n_patterns);

@@ -298,3 +298,33 @@ def test_batch_run_time_window_length():
     dev.free()
     for f in filters:
         f[0].close()
+
+
+def test_batch_many_matches_sorted_on_device():
+    # more than a few thousand matches: the batch sorts them on the device
+    say("batch with many matches")
+    sbox = fa.sbox_from_seed(81)
+    specs = [(800, 18, fa.ALPHA_DNA_GC40, fa.MODE_LDS), (1500, 20, fa.ALPHA_ASCII, fa.MODE_FOLD)]
+    filters = make_filters(sbox, specs, 500)
+    rng = np.random.default_rng(81)
+    n_texts = 40
+    route = rng.integers(0, len(filters), size=n_texts).astype(np.uint32)
+    texts = []
+    for i in range(n_texts):
+        _, b, off, _ = filters[int(route[i])]
+        alpha = fa.ALPHA_DNA_GC40 if int(route[i]) % 2 == 0 else fa.ALPHA_ASCII
+        t = fa.gen_bytes(alpha, 8100 + i, 200_000)
+        fa.plant(8200 + i, 300, t, b, off)
+        texts.append(t)
+    expect, _ = oracle_batch(sbox, filters, texts, route)
+    assert len(expect) > 8000
+    off = np.zeros(n_texts + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([t.size for t in texts])
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        assert bt.run() == len(expect)
+        assert as_rows(bt.matches()) == expect
+    dev.free()
+    for f in filters:
+        f[0].close()
