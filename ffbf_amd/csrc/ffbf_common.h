@@ -184,19 +184,48 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h)
 // Hash-table slot of a prefix-window hash pair (verification index).
 __device__ __forceinline__ uint32_t key_slot(uint32_t h1, uint32_t h2) { return mix32(h1 ^ rotl(h2, 13) ^ 0x9E3779B9u); }
 
-// Feed-forward: set the k reverse bits of every lane with `hit` in R.  Lanes
-// of the wave that target the same word are merged into one atomicOr (the
-// reference issues one atomicOr per bit, greptest.cu:86-89, 236-239).  Must be
-// called by all lanes of the wave.
-__device__ __forceinline__ void feed_forward(bool hit, uint32_t h1, uint32_t h2, uint32_t k, uint32_t mask, uint32_t *R)
+// Whether some lane 1..6 below this one in its 16-lane row also hits and has
+// the same value: drained candidates sit in queue order, so the repeats of a
+// short tandem repeat land in nearby lanes with identical hashes.
+__device__ __forceinline__ bool dup_below(bool hit, uint32_t v)
 {
+    const int h = hit ? 1 : 0;
+    bool dup = false;
+#define FFBF_DUP_AT(n)                                                                                        \
+    {                                                                                                         \
+        const uint32_t w = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + (n), 0xF, 0xF, false);    \
+        const int hh = __builtin_amdgcn_update_dpp(0, h, 0x110 + (n), 0xF, 0xF, false);                       \
+        dup = dup || (hh != 0 && w == v);                                                                     \
+    }
+    FFBF_DUP_AT(1) FFBF_DUP_AT(2) FFBF_DUP_AT(3) FFBF_DUP_AT(4) FFBF_DUP_AT(5) FFBF_DUP_AT(6)
+#undef FFBF_DUP_AT
+    return hit && dup;
+}
+
+// Feed-forward: set the k reverse bits of every lane with `hit` in R (the
+// reference issues one atomicOr per bit, greptest.cu:86-89, 236-239);
+// bit_of(j) is lane's bit index of function j.  A few hits: lanes that
+// target the same word are merged into one atomicOr, one word at a time.
+// Many hits (dense candidates): one atomicOr per lane and bit, except where
+// a lane close below sets the same bit.  Must be called by all lanes.
+template <int KMAX, class BitOf>
+__device__ __forceinline__ void feed_forward_bits(bool hit, uint32_t k, uint32_t *R, BitOf bit_of)
+{
+    const uint64_t hits = __ballot(hit);
+    if (!hits) return;
+    const bool dense = __builtin_popcountll(hits) > 4;
     const int lane = lane_id();
 #pragma unroll
-    for (int j = 0; j < 5; j++) {
+    for (int j = 0; j < KMAX; j++) {
         if (j >= (int)k) break;
-        const uint32_t idx = rev_idx(j, h1, h2) & mask;
+        const uint32_t idx = bit_of(j);
+        if (dense) {
+            const bool dup = dup_below(hit, idx);  // all lanes: DPP reads neighbours
+            if (hit && !dup) atomicOr(&R[idx >> 5], 1u << (idx & 31u));
+            continue;
+        }
         const uint32_t w = idx >> 5, bit = 1u << (idx & 31u);
-        uint64_t act = __ballot(hit);
+        uint64_t act = hits;
         while (act) {
             const int leader = __builtin_ctzll(act);
             const uint32_t w0 = __builtin_amdgcn_readlane(w, leader);
@@ -209,27 +238,17 @@ __device__ __forceinline__ void feed_forward(bool hit, uint32_t h1, uint32_t h2,
     }
 }
 
+__device__ __forceinline__ void feed_forward(bool hit, uint32_t h1, uint32_t h2, uint32_t k, uint32_t mask, uint32_t *R)
+{
+    feed_forward_bits<5>(hit, k, R, [&](int j) { return rev_idx(j, h1, h2) & mask; });
+}
+
 // The same for the split method: k (<= 11) reverse bits placed in the cache
 // or external part (setBitsInFilter_2, ffbf.cpp:205-212).
 __device__ __forceinline__ void feed_forward_split(bool hit, uint32_t h1, uint32_t h2, uint32_t k, const SplitMap &m,
                                                    uint32_t *R)
 {
-    const int lane = lane_id();
-    for (int j = 0; j < kSplitMaxK; j++) {
-        if (j >= (int)k) break;
-        const uint32_t idx = split_place(j, split_rev_fn(j, h1, h2), m);
-        const uint32_t w = idx >> 5, bit = 1u << (idx & 31u);
-        uint64_t act = __ballot(hit);
-        while (act) {
-            const int leader = __builtin_ctzll(act);
-            const uint32_t w0 = __builtin_amdgcn_readlane(w, leader);
-            const uint64_t same = __ballot(hit && w == w0) & act;
-            uint32_t bits = 0;
-            for (uint64_t mm = same; mm; mm &= mm - 1) bits |= __builtin_amdgcn_readlane(bit, __builtin_ctzll(mm));
-            if (lane == leader) atomicOr(&R[w0], bits);
-            act &= ~same;
-        }
-    }
+    feed_forward_bits<kSplitMaxK>(hit, k, R, [&](int j) { return split_place(j, split_rev_fn(j, h1, h2), m); });
 }
 
 // Appends (offset, h1, h2) of every lane with `hit`: one atomicAdd per wave,
