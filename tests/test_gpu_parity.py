@@ -263,3 +263,36 @@ def test_scan_configs_agree_on_one_input(W):
 def test_gather_bench_runs():
     ms, rate = fa.gather_bench(20, steps=4)
     assert ms > 0 and rate > 1e9
+
+
+@pytest.mark.parametrize("mode", [fa.MODE_LDS, fa.MODE_GLOBAL])
+def test_periodic_text_dense_identical_candidates(mode):
+    # tandem repeats: runs of identical windows that are all candidates,
+    # drained side by side (the feed-forward skips a bit that a nearby lane
+    # of the same drain sets; R must still hold every bit)
+    say(f"periodic text, mode {mode}")
+    sbox = fa.sbox_from_seed(61)
+    W = 19
+    rng = np.random.default_rng(61)
+    units = [b"A", b"AC", b"ACG", b"ACGT", b"AACGT", b"AACCGT"]
+    parts, pats = [], []
+    for i in range(300):
+        u = units[i % len(units)]
+        run = (u * (600 // len(u) + 1))[: int(rng.integers(100, 600))]
+        parts.append(run)
+        parts.append(fa.gen_bytes(fa.ALPHA_DNA_GC40, 6100 + i, int(rng.integers(50, 400))).tobytes())
+        if i < 40:
+            off0 = int(rng.integers(0, len(u)))
+            pats.append((u * 40)[off0:off0 + W + int(rng.integers(0, 20))])
+    text = np.frombuffer(b"".join(parts), dtype=np.uint8).copy()
+    b0, o0 = fa.gen_patterns(fa.ALPHA_DNA_GC40, 6200, 3000, W, 50)
+    pats += [bytes(b0[int(o0[i]):int(o0[i + 1])]) for i in range(o0.size - 1)]
+    b, off = fa.patterns_from_list(pats)
+    for log2 in (14, 20):  # dense false positives, then mostly true repeats
+        r = Oracle(W, 4, log2, sbox).run(text, b, off)
+        with fa.Matcher(W, 4, log2, sbox) as m:
+            m.set_patterns(b, off)
+            m.build()
+            m.set_scan_mode(mode if (mode != fa.MODE_LDS or log2 <= 20) else fa.MODE_AUTO)
+            assert_same(gpu_all(m, text), r, m.filter(fa.FILTER_FORWARD))
+        assert r["cand"].size > 5000
