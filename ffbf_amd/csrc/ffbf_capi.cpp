@@ -702,6 +702,9 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
     c->n_pat = n;
     c->n_long = n_long;
     c->patterns = true;
+    // F was built from the previous set: a scan needs ffbf_build (or a
+    // filter load) for this one first
+    c->built = false;
     c->scanned = c->passed = c->verified = false;
     return FFBF_OK;
 }

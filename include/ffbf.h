@@ -128,7 +128,8 @@ void ffbf_destroy(ffbf_ctx *ctx);
 /* Loads a pattern set: pattern i is bytes[offsets[i] .. offsets[i+1]);
  * offsets has n_patterns+1 entries.  Copied to the device (resident).
  * Patterns shorter than W are kept (their ids count) but never match,
- * as ffbf.cpp:271-275 skips them. */
+ * as ffbf.cpp:271-275 skips them.  Loading a set discards the built filter:
+ * scans return FFBF_ESTATE until ffbf_build or ffbf_filter_load. */
 int ffbf_set_patterns(ffbf_ctx *ctx, const uint8_t *bytes, const uint64_t *offsets, uint64_t n_patterns);
 
 /* Same, from newline-separated text (one pattern per line, id = 0-based line

@@ -296,3 +296,22 @@ def test_periodic_text_dense_identical_candidates(mode):
             m.set_scan_mode(mode if (mode != fa.MODE_LDS or log2 <= 20) else fa.MODE_AUTO)
             assert_same(gpu_all(m, text), r, m.filter(fa.FILTER_FORWARD))
         assert r["cand"].size > 5000
+
+
+def test_new_pattern_set_needs_a_build():
+    say("pattern set replaced after a build")
+    sbox = fa.sbox_from_seed(62)
+    b1, o1, text = make_case(fa.ALPHA_ASCII, 19, 500, 19, 40, 200_000, 50, 63)
+    b2, o2 = fa.gen_patterns(fa.ALPHA_ASCII, 64, 700, 19, 40)
+    fa.plant(65, 40, text, b2, o2)
+    with fa.Matcher(19, 4, 18, sbox) as m:
+        m.set_patterns(b1, o1)
+        m.build()
+        m.scan(text)
+        m.set_patterns(b2, o2)
+        with pytest.raises(fa.FFBFError) as e:
+            m.scan(text)
+        assert e.value.status == fa.ESTATE
+        m.build()
+        r = Oracle(19, 4, 18, sbox).run(text, b2, o2)
+        assert_same(gpu_all(m, text), r, m.filter())
