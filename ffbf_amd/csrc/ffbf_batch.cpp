@@ -57,6 +57,7 @@ struct ffbf_batch {
     std::vector<BatchText> h_texts;
     bool have_texts = false;
 
+    std::vector<uint64_t> gens;  // the filters' generations the descriptors were built from
     ffbf_batch_tuning tuning{32u, 0u, 1024u, 1u};
     uint32_t probes_eff = 1;
     hipGraph_t graph = nullptr;
@@ -308,6 +309,42 @@ int check_batch(ffbf_batch *b)
     return FFBF_OK;
 }
 
+// Cuts the current texts into chunks again (new chunk size, or filters whose
+// prefilter class changed).  Texts are contiguous ranges of one buffer, so
+// their offsets follow from their starts and sizes.
+int rechunk_texts(ffbf_batch *b)
+{
+    if (!b->have_texts) return FFBF_OK;
+    std::vector<uint64_t> off(b->n_texts + 1, 0);
+    std::vector<uint32_t> fil(b->n_texts);
+    for (uint32_t i = 0; i < b->n_texts; i++) {
+        off[i] = b->h_texts[i].start;
+        off[i + 1] = b->h_texts[i].start + b->h_texts[i].n;
+        fil[i] = b->h_texts[i].filter;
+    }
+    return ffbf_batch_set_texts(b, b->texts, off.data(), fil.data(), b->n_texts);
+}
+
+// A borrowed context was rebuilt, reloaded or given another scan mode since
+// the descriptors were made: rebuild them, and cut the texts again (the
+// chunk order depends on each filter's prefilter class).
+// A run or new texts that follow need no re-cut (rechunk false).
+int sync_filters(ffbf_batch *b, bool rechunk)
+{
+    bool changed = b->gens.size() != b->filters.size();
+    for (size_t f = 0; !changed && f < b->filters.size(); f++) changed = b->gens[f] != b->filters[f]->generation;
+    if (!changed) return FFBF_OK;
+    for (ffbf_ctx *c : b->filters) {
+        if (!c->built || !c->patterns) return fail(FFBF_ESTATE, "a filter of the batch has no built filter");
+        HIP_TRY(hipStreamSynchronize(c->stream));  // a rebuild may still be running
+    }
+    drop_graph(b);
+    RET_IF(refresh_filters(b));
+    b->gens.resize(b->filters.size());
+    for (size_t f = 0; f < b->filters.size(); f++) b->gens[f] = b->filters[f]->generation;
+    return rechunk ? rechunk_texts(b) : FFBF_OK;
+}
+
 }  // namespace
 
 namespace ffbf {
@@ -380,9 +417,7 @@ int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch *
         RET_IF(b->cand.ensure(1 << 16, "batch candidates"));
         RET_IF(b->cstage.ensure(b->n_cu, b->cand.cap));
         RET_IF(b->matches.ensure(1 << 16, "batch matches"));
-        // the filters' own streams may still be building
-        for (ffbf_ctx *c : b->filters) HIP_TRY(hipStreamSynchronize(c->stream));
-        return refresh_filters(b);
+        return sync_filters(b, false);  // also waits for the filters' own streams
     };
     const int st = init();
     if (st != FFBF_OK) {
@@ -430,6 +465,7 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     RET_IF(check_batch(b));
     if (!offsets || (n_texts && (!filter_of_text || !dev_texts))) return fail(FFBF_EINVAL, "null argument");
     if (n_texts >= (1u << (64 - kBatchOffsetBits))) return fail(FFBF_EINVAL, "too many texts in one batch");
+    RET_IF(sync_filters(b, false));
     const uint32_t nf = (uint32_t)b->filters.size();
     std::vector<BatchText> tx(n_texts);
     uint64_t total = 0, maxn = 0;
@@ -523,6 +559,7 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
 int ffbf_batch_run(ffbf_batch *b, uint64_t *n_matches)
 {
     RET_IF(check_batch(b));
+    RET_IF(sync_filters(b, true));
     if (!b->have_texts) return fail(FFBF_ESTATE, "no texts set");
     for (int attempt = 0; attempt < 3; attempt++) {
         RET_IF(run_once(b));
@@ -579,6 +616,7 @@ int ffbf_batch_get_matches(ffbf_batch *b, ffbf_batch_match *out, uint64_t cap, u
 int ffbf_batch_run_per_text(ffbf_batch *b, uint64_t *n_matches)
 {
     RET_IF(check_batch(b));
+    RET_IF(sync_filters(b, true));
     if (!b->have_texts) return fail(FFBF_ESTATE, "no texts set");
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<BatchMatchRec> all;
@@ -609,24 +647,14 @@ int ffbf_batch_set_tuning(ffbf_batch *b, const ffbf_batch_tuning *t)
     if (t->block != 0 && t->block != 512 && t->block != 1024) return fail(FFBF_EINVAL, "block must be 0, 512 or 1024");
     if (t->probes > b->k) return fail(FFBF_EINVAL, "probes must be 0..k");
     if (t->chunk_iters > (1u << 20)) return fail(FFBF_EINVAL, "chunk_iters too large");
+    RET_IF(sync_filters(b, true));
     const bool rechunk = (t->chunk_iters ? t->chunk_iters : 32u) != (b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u);
     b->tuning = *t;
     if (!b->tuning.block) b->tuning.block = 1024;
     if (!b->tuning.chunk_iters) b->tuning.chunk_iters = 32;
     drop_graph(b);
     RET_IF(refresh_filters(b));
-    if (rechunk && b->have_texts) {
-        // re-cut the current texts with the new chunk size
-        std::vector<uint64_t> off(b->n_texts + 1);
-        std::vector<uint32_t> fil(b->n_texts);
-        for (uint32_t i = 0; i < b->n_texts; i++) {
-            off[i] = b->h_texts[i].start;
-            off[i + 1] = b->h_texts[i].start + b->h_texts[i].n;
-            fil[i] = b->h_texts[i].filter;
-        }
-        // texts are contiguous ranges of one buffer; rebuild offsets from starts
-        RET_IF(ffbf_batch_set_texts(b, b->texts, off.data(), fil.data(), b->n_texts));
-    }
+    if (rechunk) RET_IF(rechunk_texts(b));  // re-cut the current texts with the new chunk size
     return FFBF_OK;
 }
 
@@ -643,6 +671,7 @@ int ffbf_batch_autotune(ffbf_batch *b, ffbf_batch_tuning *best, float *best_ms)
 {
     RET_IF(check_batch(b));
     if (!b->have_texts) return fail(FFBF_ESTATE, "autotune needs a batch of texts");
+    RET_IF(sync_filters(b, true));
     const ffbf_batch_tuning start = b->tuning;
     ffbf_batch_tuning top = start;
     float top_ms = 1e30f;

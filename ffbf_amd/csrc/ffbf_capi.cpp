@@ -165,6 +165,7 @@ int choose_mode_split(ffbf_ctx *c)
 
 int choose_mode(ffbf_ctx *c)
 {
+    c->generation++;
     if (c->split) return choose_mode_split(c);
     const uint32_t max_words = ffbf::max_fold_words(c->device);
     const uint32_t k = c->params.k;
@@ -705,6 +706,7 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
     // F was built from the previous set: a scan needs ffbf_build (or a
     // filter load) for this one first
     c->built = false;
+    c->generation++;
     c->scanned = c->passed = c->verified = false;
     return FFBF_OK;
 }

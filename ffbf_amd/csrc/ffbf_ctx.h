@@ -129,6 +129,10 @@ struct ffbf_ctx {
     ffbf::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned
 
+    // bumped whenever F, the pattern set or the scan mode changes: a batch
+    // that borrows the context rebuilds its descriptors when it sees a new value
+    uint64_t generation = 0;
+
     // scan mode and tuning
     int mode_req = -1;
     uint32_t fold_log2_req = 0;

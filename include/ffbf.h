@@ -253,8 +253,11 @@ void *ffbf_get_stream(ffbf_ctx *ctx);
  * each routed to one of them, in one persistent launch replayed from a
  * captured hipGraph.  Each text's match list equals what ffbf_run_device
  * gives for it alone.  A filter's R after a batch run is the union over the
- * texts routed to it.  While a batch is in use its contexts must not be used
- * directly. */
+ * texts routed to it.  While a batch call runs its contexts must not be used
+ * directly.  Between calls they may be rebuilt, given new patterns or another
+ * scan mode: the next batch call picks the change up (a context whose new
+ * pattern set is not yet built makes it fail with FFBF_ESTATE).  A context
+ * must outlive every batch that holds it. */
 typedef struct ffbf_batch ffbf_batch;
 
 typedef struct ffbf_batch_match {

@@ -328,3 +328,43 @@ def test_batch_many_matches_sorted_on_device():
     dev.free()
     for f in filters:
         f[0].close()
+
+
+def test_batch_follows_changes_to_its_filters():
+    """A batch borrows its filters: a new scan mode (which moves a filter to
+    the other prefilter class), a new pattern set with a rebuild, or a set
+    not yet built must be seen by the next run."""
+    say("batch after its filters change")
+    sbox = fa.sbox_from_seed(81)
+    filters = make_filters(sbox, SPECS[:3], 600)
+    texts, off, route = make_texts(filters, 30, 9, max_len=120_000)
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    expect, _ = oracle_batch(sbox, filters, texts, route)
+    with fa.Batch([f[0] for f in filters]) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+        # LDS filter -> global, global filter -> LDS fold (graph replayed before)
+        filters[0][0].set_scan_mode(fa.MODE_GLOBAL)
+        filters[2][0].set_scan_mode(fa.MODE_FOLD)
+        say("  modes swapped")
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+        bt.run_per_text()
+        assert as_rows(bt.matches()) == expect
+        # new patterns for filter 1, not yet built: refused
+        m1, _, _, log2 = filters[1]
+        b2, o2 = fa.gen_patterns(fa.ALPHA_ASCII, 601, 4000, W, 60)
+        m1.set_patterns(b2, o2)
+        with pytest.raises(fa.FFBFError) as e:
+            bt.run()
+        assert e.value.status == fa.ESTATE
+        m1.build()
+        say("  filter 1 rebuilt with a new set")
+        filters[1] = (m1, b2, o2, log2)
+        expect2, _ = oracle_batch(sbox, filters, texts, route)
+        bt.run()
+        assert as_rows(bt.matches()) == expect2
+    dev.free()
+    for f in filters:
+        f[0].close()
