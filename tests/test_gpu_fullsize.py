@@ -194,3 +194,66 @@ def test_full_size_split_1m_patterns_2gib_lines():
             say(f"  mode request {mode}: scan mode {m.timings()['scan_mode']}")
             assert_same(gpu_staged(m, dev, N_TEXT), r)
     dev.free()
+
+
+@pytest.mark.parametrize("name", ["batch-4096texts-1set", "batch-2048texts-64sets"])
+def test_full_size_batches(name):
+    """The two benchmarked batches in full (4,096 ASCII texts, 1.5 GiB, one
+    100k-pattern set; 2,048 DNA texts with tandem repeats, 3.3 GiB, 64 sets
+    of 1k..1M patterns): every text's matches against the oracle's flow
+    (scan, pattern pass, verification) on that text alone, and each filter's
+    R against the union of its texts' reverse filters."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from bench import batch_inputs
+
+    t0 = time.perf_counter()
+    d = batch_inputs(fa, name)
+    say(f"{name}: {d['n_texts']} texts, {d['bytes'] >> 20} MiB, generated in {time.perf_counter() - t0:.1f} s")
+    W, K, sbox, texts, off, route = d["W"], d["k"], d["sbox"], d["texts"], d["offsets"], d["route"]
+    t0 = time.perf_counter()
+    orc = [Oracle(W, K, log2, sbox) for _, _, log2 in d["sets"]]
+    Fs = [o.build(pb, po) for o, (pb, po, _) in zip(orc, d["sets"])]
+
+    def one(i):
+        f = int(route[i])
+        pb, po, _ = d["sets"][f]
+        t = texts[int(off[i]):int(off[i + 1])]
+        R, cand = orc[f].scan(t, Fs[f])
+        ids, offs = orc[f].verify(t, pb, po, cand, orc[f].pattern_pass(pb, po, R))
+        return f, R, ids, offs
+
+    rows, R_union = [], {}
+    with ThreadPoolExecutor(THREADS) as ex:
+        for lo in range(0, d["n_texts"], 256):
+            for i, (f, R, ids, offs) in zip(range(lo, d["n_texts"]), ex.map(one, range(lo, min(lo + 256, d["n_texts"])))):
+                rows.append(np.rec.fromarrays([np.full(ids.size, i, np.uint32), ids, offs]))
+                R_union[f] = R if f not in R_union else (R_union[f] | R)
+    expect = np.concatenate(rows)
+    expect.sort(order=["f0", "f1", "f2"])
+    say(f"  oracle on {THREADS} threads: {time.perf_counter() - t0:.1f} s, {expect.size} matches")
+    assert expect.size >= d["n_texts"]
+    dev = fa.DeviceBuffer.from_host(texts)
+    ms = []
+    for pb, po, log2 in d["sets"]:
+        m = fa.Matcher(W, K, log2, sbox)
+        m.set_patterns(pb, po)
+        m.build()
+        ms.append(m)
+    with fa.Batch(ms) as bt:
+        bt.set_texts(dev.ptr, off, route)
+        for use_graph in (True, False):
+            bt.set_tuning(use_graph=use_graph)
+            n = bt.run()
+            got = bt.matches()
+            say(f"  graph={use_graph}: {n} matches")
+            assert n == expect.size
+            assert np.array_equal(got["text_id"], expect["f0"]), "text ids differ"
+            assert np.array_equal(got["pattern_id"], expect["f1"]), "pattern ids differ"
+            assert np.array_equal(got["offset"], expect["f2"]), "offsets differ"
+            for f, m in enumerate(ms):
+                R = m.filter(fa.FILTER_REVERSE)
+                assert np.array_equal(R, R_union[f]) if f in R_union else not R.any(), f"R of filter {f}"
+    for m in ms:
+        m.close()
+    dev.free()
