@@ -476,6 +476,24 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
     }
 
 
+def copy_kernel_rate(fa, nbytes=4 << 30, reps=5):
+    """HBM read+write GB/s of the library's plain copy kernel (median of
+    `reps` synchronous copies of `nbytes`): the practical streaming ceiling
+    beside the 8 TB/s peak."""
+    import numpy as np
+
+    src, dst = fa.DeviceBuffer(nbytes), fa.DeviceBuffer(nbytes)
+    dst.copy_from(src, nbytes)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dst.copy_from(src, nbytes)
+        ts.append(time.perf_counter() - t0)
+    src.free()
+    dst.free()
+    return round(2 * nbytes / float(np.median(ts)) / 1e9, 1)
+
+
 def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
     """Oracle (C restatement of the reference path) on a bounded text sample."""
     from oracle import Oracle
@@ -741,6 +759,7 @@ def main() -> int:
     st = np.mean(np.array(stages), axis=0)
     mode = m.timings()
     ceiling = probe_ceiling(fa, m, k, n_text, W, t_scan)
+    copy_gbs = copy_kernel_rate(fa)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -766,6 +785,7 @@ def main() -> int:
             "traffic": traffic_from_profiles(args.workload),
             "avg_launch_ms": round(t_scan, 4),
             "algorithmic_bytes": n_text,
+            "copy_kernel_gbs": copy_gbs,
         },
         "stages_ms": {"build": round(st[0], 4), "scan": round(st[1], 4), "pattern_pass": round(st[2], 4),
                       "verify": round(st[3], 4)},
