@@ -174,10 +174,12 @@ def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0):
     batched_s = (time.perf_counter() - t0) / steps
     run_ms, _ = bt.timings()
     # scan kernel alone: one stage-by-stage run
-    bt.set_tuning(best["chunk_iters"], best["probes"], best["block"], use_graph=False)
+    bt.set_tuning(best["chunk_iters"], best["probes"], best["block"], use_graph=False,
+                  bytes_per_lane=best["bytes_per_lane"])
     bt.run()
     _, scan_ms = bt.timings()
-    bt.set_tuning(best["chunk_iters"], best["probes"], best["block"], use_graph=True)
+    bt.set_tuning(best["chunk_iters"], best["probes"], best["block"], use_graph=True,
+                  bytes_per_lane=best["bytes_per_lane"])
     bt.run_per_text()
     t0 = time.perf_counter()
     for _ in range(per_text_reps):

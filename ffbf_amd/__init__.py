@@ -104,6 +104,7 @@ class BatchTuning(ctypes.Structure):
         ("probes", ctypes.c_uint32),
         ("block", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
+        ("bytes_per_lane", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -625,8 +626,9 @@ class Batch:
         _check(_lib.ffbf_batch_get_matches(self._h, _ptr(out), n.value, _c.byref(n)))
         return out
 
-    def set_tuning(self, chunk_iters: int = 0, probes: int = 0, block: int = 0, use_graph: bool = True) -> None:
-        t = BatchTuning(chunk_iters, probes, block, 1 if use_graph else 0)
+    def set_tuning(self, chunk_iters: int = 0, probes: int = 0, block: int = 0, use_graph: bool = True,
+                   bytes_per_lane: int = 0) -> None:
+        t = BatchTuning(chunk_iters, probes, block, 1 if use_graph else 0, bytes_per_lane)
         _check(_lib.ffbf_batch_set_tuning(self._h, _c.byref(t)))
 
     def tuning(self) -> dict:

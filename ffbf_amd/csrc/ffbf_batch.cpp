@@ -58,7 +58,7 @@ struct ffbf_batch {
     bool have_texts = false;
 
     std::vector<uint64_t> gens;  // the filters' generations the descriptors were built from
-    ffbf_batch_tuning tuning{32u, 0u, 1024u, 1u};
+    ffbf_batch_tuning tuning{32u, 0u, 1024u, 1u, 0u};
     uint32_t probes_eff = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -82,6 +82,20 @@ void drop_graph(ffbf_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+// Whether the LDS-class scan has a 32-bytes-per-lane instance: the pair
+// method at a compile-time W.
+bool takes_32_bytes(const ffbf_batch *b)
+{
+    return !b->filters[0]->split && (b->W == 19 || b->W == 24 || b->W == 32);
+}
+
+// 16-byte units per lane of the LDS-class scan: two (32 window starts) where
+// there is an instance for it, unless 16 bytes per lane were asked for.
+uint32_t units_per_lane(const ffbf_batch *b)
+{
+    return (takes_32_bytes(b) && b->tuning.bytes_per_lane != 16u) ? 2u : 1u;
 }
 
 uint32_t bits_for(uint64_t v)
@@ -220,7 +234,8 @@ int enqueue_run(ffbf_batch *b, bool timed)
     ba.method = b->filters[0]->split ? METHOD_SPLIT : METHOD_PAIR;
     if (b->has_lds) {
         ba.ctl = b->ctl.p;
-        HIP_TRY(launch_scan_batch(ba, b->tuning.block, b->probes_eff, b->pre_words_max, b->n_cu, b->stream));
+        HIP_TRY(launch_scan_batch(ba, b->tuning.block, b->probes_eff, units_per_lane(b), b->pre_words_max, b->n_cu,
+                                  b->stream));
     }
     if (b->has_global) {
         ba.ctl = b->ctl.p + 4;
@@ -480,8 +495,9 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
         total += tx[i].n;
         maxn = std::max(maxn, tx[i].n);
     }
-    // chunks of chunk_iters wave-iterations (kIterUnits units of 16 bytes),
-    // ordered by chunk class, filter, then text
+    // chunks of chunk_iters wave-iterations (kIterUnits units of 16 bytes, or
+    // kIterUnits2 in the LDS class at two units per lane), ordered by chunk
+    // class, filter, then text
     std::vector<uint32_t> order(n_texts);
     for (uint32_t i = 0; i < n_texts; i++) order[i] = i;
     // chunk class of a filter: 0 when its prefilter is held in LDS
@@ -501,7 +517,8 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
         const uint64_t npos = tx[i].n >= b->W ? tx[i].n - b->W + 1 : 0;
         if (!npos) continue;
         const uint64_t units = (lead + npos + 15u) >> 4;
-        const uint64_t iters = (units + kIterUnits - 1u) / kIterUnits;
+        const uint64_t iu = (cls(tx[i].filter) == 0u && units_per_lane(b) == 2u) ? kIterUnits2 : kIterUnits;
+        const uint64_t iters = (units + iu - 1u) / iu;
         if (iters > 0xFFFFFFFFull) return fail(FFBF_EINVAL, "text too long for one batch");
         for (uint64_t it = 0; it < iters; it += ci) {
             BatchChunk c;
@@ -647,14 +664,19 @@ int ffbf_batch_set_tuning(ffbf_batch *b, const ffbf_batch_tuning *t)
     if (t->block != 0 && t->block != 512 && t->block != 1024) return fail(FFBF_EINVAL, "block must be 0, 512 or 1024");
     if (t->probes > b->k) return fail(FFBF_EINVAL, "probes must be 0..k");
     if (t->chunk_iters > (1u << 20)) return fail(FFBF_EINVAL, "chunk_iters too large");
+    if (t->bytes_per_lane != 0 && t->bytes_per_lane != 16 && t->bytes_per_lane != 32)
+        return fail(FFBF_EINVAL, "bytes_per_lane must be 0, 16 or 32");
     RET_IF(sync_filters(b, true));
-    const bool rechunk = (t->chunk_iters ? t->chunk_iters : 32u) != (b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u);
+    const uint32_t upl_before = units_per_lane(b);
+    const bool new_chunk =
+        (t->chunk_iters ? t->chunk_iters : 32u) != (b->tuning.chunk_iters ? b->tuning.chunk_iters : 32u);
     b->tuning = *t;
+    const bool rechunk = new_chunk || units_per_lane(b) != upl_before;
     if (!b->tuning.block) b->tuning.block = 1024;
     if (!b->tuning.chunk_iters) b->tuning.chunk_iters = 32;
     drop_graph(b);
     RET_IF(refresh_filters(b));
-    if (rechunk) RET_IF(rechunk_texts(b));  // re-cut the current texts with the new chunk size
+    if (rechunk) RET_IF(rechunk_texts(b));  // re-cut the current texts with the new chunk geometry
     return FFBF_OK;
 }
 
@@ -664,6 +686,7 @@ int ffbf_batch_get_tuning(ffbf_batch *b, ffbf_batch_tuning *t)
     if (!t) return fail(FFBF_EINVAL, "null tuning");
     *t = b->tuning;
     t->probes = b->probes_eff;
+    t->bytes_per_lane = 16u * units_per_lane(b);
     return FFBF_OK;
 }
 
@@ -678,18 +701,24 @@ int ffbf_batch_autotune(ffbf_batch *b, ffbf_batch_tuning *best, float *best_ms)
     const uint32_t chunk_opts[] = {16u, 32u, 64u, 128u};
     const uint32_t block_opts[] = {1024u, 512u};
     const bool any_lds = b->pre_words_max > 0;
-    for (uint32_t blk : block_opts) {
-        for (uint32_t p = 1; p <= std::min(3u, b->k); p++) {
-            if (!any_lds && p > 1) break;
-            for (uint32_t ci : chunk_opts) {
-                ffbf_batch_tuning t{ci, p, blk, start.use_graph};
-                RET_IF(ffbf_batch_set_tuning(b, &t));
-                uint64_t nm = 0;
-                RET_IF(ffbf_batch_run(b, &nm));  // warm (and capture)
-                RET_IF(ffbf_batch_run(b, &nm));
-                if (b->run_ms < top_ms) {
-                    top_ms = b->run_ms;
-                    top = t;
+    // 32 bytes per lane only where the LDS-class scan can take it
+    const bool wide = any_lds && takes_32_bytes(b);
+    const uint32_t bpl_opts[] = {32u, 16u};
+    for (uint32_t bpl : bpl_opts) {
+        if (bpl == 32u && !wide) continue;
+        for (uint32_t blk : block_opts) {
+            for (uint32_t p = 1; p <= std::min(3u, b->k); p++) {
+                if (!any_lds && p > 1) break;
+                for (uint32_t ci : chunk_opts) {
+                    ffbf_batch_tuning t{ci, p, blk, start.use_graph, bpl};
+                    RET_IF(ffbf_batch_set_tuning(b, &t));
+                    uint64_t nm = 0;
+                    RET_IF(ffbf_batch_run(b, &nm));  // warm (and capture)
+                    RET_IF(ffbf_batch_run(b, &nm));
+                    if (b->run_ms < top_ms) {
+                        top_ms = b->run_ms;
+                        top = t;
+                    }
                 }
             }
         }

@@ -4,7 +4,8 @@
 // The reference scans one text against one filter per run
 // (greptest.cu:505-576).  Here the texts of a batch sit back to back in one
 // device buffer, each routed to one filter; the host cuts every text into
-// chunks of wave-iterations (62 units of 16 bytes), ordered by filter, and persistent workgroups
+// chunks of wave-iterations (62 units of 16 bytes, or 126 with two units
+// per lane), ordered by filter, and persistent workgroups
 // pull chunks from a global counter.  A workgroup reloads its LDS prefilter
 // only when the next chunk belongs to another filter, and drains its survivor
 // queues before it does.  Filters whose prefilter stays in global memory are
@@ -17,14 +18,32 @@ namespace ffbf {
 
 namespace {
 
+// A record read at a wave-uniform address, kept in scalar registers: every
+// 32-bit word is taken from the first active lane.
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T &v)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T out;
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(&v);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) d[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)s[i]);
+    return out;
+}
+
 // GLOBAL_ONLY: every filter of the launch keeps its prefilter in global
 // memory (F, or an L2-sized fold); no LDS is reserved for one, so small
 // blocks fit many waves per CU to hide the probes' round trips.
 // SPLIT: filters of the CPU binary's method (64-bit hash from a (lo, hi)
 // sbox, windows holding a newline never tested, cache-part fold).
-template <int WT, int BLOCK, int P, bool GLOBAL_ONLY, bool SPLIT = false>
+// UPL: 16-byte units per lane (2: 32 window starts per lane, chunks counted
+// in wave-iterations of kIterUnits2 units; compile-time W, pair method, LDS
+// class only).
+template <int WT, int BLOCK, int P, bool GLOBAL_ONLY, bool SPLIT = false, int UPL = 1>
 __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
 {
+    static_assert(UPL == 1 || (UPL == 2 && WT > 0 && !SPLIT && !GLOBAL_ONLY), "two units per lane: LDS class, pair");
     using QG = QueueGeom<BLOCK>;
     constexpr uint32_t kSboxWords = SPLIT ? 512u : 256u;
     extern __shared__ __align__(16) uint32_t s_pre[];
@@ -54,7 +73,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         __syncthreads();
         const uint32_t ci = __builtin_amdgcn_readfirstlane(s_next);
         if (ci >= n_chunks) break;
-        const BatchChunk c = b.chunks[chunk0 + ci];
+        const BatchChunk c = wave_uniform(b.chunks[chunk0 + ci]);
         if (c.filter != loaded) {
             if (qn) {
                 wave_sync();
@@ -63,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
                 qn = 0;
             }
             if (loaded != 0xFFFFFFFFu) flush_pending(pend, a);
-            a = b.filters[c.filter];
+            a = wave_uniform(b.filters[c.filter]);
             __syncthreads();
             if constexpr (lds) {
                 const uint4 *src = (const uint4 *)a.pre;
@@ -73,7 +92,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
             __syncthreads();
             loaded = c.filter;
         }
-        const BatchText t = b.tdesc[c.text];
+        const BatchText t = wave_uniform(b.tdesc[c.text]);
         const uint8_t *text = b.texts + t.start;
         const uint64_t lead = (uint64_t)((uintptr_t)text & 15u);
         const uint8_t *base = text - lead;
@@ -83,6 +102,24 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         uint64_t it = (uint64_t)c.it0 + (uint64_t)wv;
 
         uint4 cur = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (UPL == 2) {
+            // no load ahead: the 8 registers it holds would make the P = 3
+            // instances spill at 1024 threads
+            for (; it < it_end; it += BLOCK / 64) {
+                const uint64_t u = it * kIterUnits2 + 2u * (uint64_t)lane_id();
+                const uint4 u0 = load_chunk(base, lead, t.n, u);
+                const uint4 u1 = load_chunk(base, lead, t.n, u + 1u);
+                HashPair32<(WT > 0 ? WT : 1)> hp;
+                hp.load(u0, u1, s_sbox);
+                uint32_t h1[kUnit], h2[kUnit];
+                hp.first(h1, h2);
+                const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
+                test_and_queue<true, BLOCK, P, false, 63>(a, s_pre, h1, h2, p0, npos, tag, q, qn, pend);
+                hp.second(h1, h2);
+                test_and_queue<true, BLOCK, P, false, 63>(a, s_pre, h1, h2, p0 + 16, npos, tag, q, qn, pend);
+            }
+            continue;
+        }
         if constexpr (WT > 0) {
             if (it < it_end) cur = load_chunk(base, lead, t.n, it * kIterUnits + (uint64_t)lane_id());
         }
@@ -144,39 +181,45 @@ __global__ __launch_bounds__(256) void k_verify_batch(BatchVerifyArgs v)
     }
 }
 
-template <int WT, int BLOCK, int P, bool GLOBAL_ONLY = false, bool SPLIT = false>
+template <int WT, int BLOCK, int P, bool GLOBAL_ONLY = false, bool SPLIT = false, int UPL = 1>
 hipError_t launch_batch_t(const BatchArgs &a, uint32_t pre_words_max, int n_cu, hipStream_t s)
 {
     const size_t dyn = GLOBAL_ONLY ? 0u : (size_t)pre_words_max * 4u;
     if (dyn > 0) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT, UPL>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
     }
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT>,
-                                                                BLOCK, dyn);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT, UPL>, BLOCK, dyn);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT>), dim3((uint32_t)(n_cu * per_cu)), dim3(BLOCK),
-                       dyn, s, a);
+    hipLaunchKernelGGL((k_scan_batch<WT, BLOCK, P, GLOBAL_ONLY, SPLIT, UPL>), dim3((uint32_t)(n_cu * per_cu)),
+                       dim3(BLOCK), dyn, s, a);
     return hipGetLastError();
 }
 
-template <int WT, int BLOCK>
+template <int WT, int BLOCK, int UPL>
 hipError_t launch_batch_p(const BatchArgs &a, uint32_t probes, uint32_t pre_words_max, int n_cu, hipStream_t s)
 {
-    if (probes <= 1u) return launch_batch_t<WT, BLOCK, 1>(a, pre_words_max, n_cu, s);
-    if (probes == 2u) return launch_batch_t<WT, BLOCK, 2>(a, pre_words_max, n_cu, s);
-    return launch_batch_t<WT, BLOCK, 3>(a, pre_words_max, n_cu, s);
+    if (probes <= 1u) return launch_batch_t<WT, BLOCK, 1, false, false, UPL>(a, pre_words_max, n_cu, s);
+    if (probes == 2u) return launch_batch_t<WT, BLOCK, 2, false, false, UPL>(a, pre_words_max, n_cu, s);
+    return launch_batch_t<WT, BLOCK, 3, false, false, UPL>(a, pre_words_max, n_cu, s);
 }
 
 template <int WT>
-hipError_t launch_batch_w(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
-                          hipStream_t s)
+hipError_t launch_batch_w(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t upl, uint32_t pre_words_max,
+                          int n_cu, hipStream_t s)
 {
-    if (block == 512u) return launch_batch_p<WT, 512>(a, probes, pre_words_max, n_cu, s);
-    return launch_batch_p<WT, 1024>(a, probes, pre_words_max, n_cu, s);
+    if constexpr (WT > 0) {
+        if (upl == 2u) {
+            if (block == 512u) return launch_batch_p<WT, 512, 2>(a, probes, pre_words_max, n_cu, s);
+            return launch_batch_p<WT, 1024, 2>(a, probes, pre_words_max, n_cu, s);
+        }
+    }
+    if (block == 512u) return launch_batch_p<WT, 512, 1>(a, probes, pre_words_max, n_cu, s);
+    return launch_batch_p<WT, 1024, 1>(a, probes, pre_words_max, n_cu, s);
 }
 
 // Split filters: 1024-thread blocks for the LDS class (any requested block
@@ -191,17 +234,17 @@ hipError_t launch_batch_split(const BatchArgs &a, uint32_t probes, uint32_t pre_
 
 }  // namespace
 
-hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
-                             hipStream_t s)
+hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t units_per_lane,
+                             uint32_t pre_words_max, int n_cu, hipStream_t s)
 {
     if (a.method == METHOD_SPLIT)
         return a.W == 19 ? launch_batch_split<19>(a, probes, pre_words_max, n_cu, s)
                          : launch_batch_split<0>(a, probes, pre_words_max, n_cu, s);
     switch (a.W) {
-    case 19: return launch_batch_w<19>(a, block, probes, pre_words_max, n_cu, s);
-    case 24: return launch_batch_w<24>(a, block, probes, pre_words_max, n_cu, s);
-    case 32: return launch_batch_w<32>(a, block, probes, pre_words_max, n_cu, s);
-    default: return launch_batch_w<0>(a, block, probes, pre_words_max, n_cu, s);
+    case 19: return launch_batch_w<19>(a, block, probes, units_per_lane, pre_words_max, n_cu, s);
+    case 24: return launch_batch_w<24>(a, block, probes, units_per_lane, pre_words_max, n_cu, s);
+    case 32: return launch_batch_w<32>(a, block, probes, units_per_lane, pre_words_max, n_cu, s);
+    default: return launch_batch_w<0>(a, block, probes, units_per_lane, pre_words_max, n_cu, s);
     }
 }
 

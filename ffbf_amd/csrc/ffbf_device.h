@@ -118,6 +118,9 @@ constexpr int kBatchOffsetBits = 40;
 // one per lane, and owns the windows that start in the first kIterUnits of
 // them; the top lanes' units only complete the windows of the lanes below.
 constexpr uint32_t kIterUnits = 62;
+// Two units per lane (32 window starts): a wave-iteration owns the windows of
+// the first 126 of 128 units; lane 63's pair only completes lane 62's windows.
+constexpr uint32_t kIterUnits2 = 126;
 
 struct BatchArgs {
     const uint8_t *texts;
@@ -211,8 +214,11 @@ hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave
                                unsigned long long *counters, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
 
-hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t pre_words_max, int n_cu,
-                             hipStream_t s);
+// units_per_lane 2 (32 window starts per lane) applies to compile-time W of
+// the pair method; chunks of the LDS class are then counted in wave-iterations
+// of kIterUnits2 units
+hipError_t launch_scan_batch(const BatchArgs &a, uint32_t block, uint32_t probes, uint32_t units_per_lane,
+                             uint32_t pre_words_max, int n_cu, hipStream_t s);
 // the same over filters whose prefilter is in global memory (256-thread blocks, no LDS prefilter)
 hipError_t launch_scan_batch_global(const BatchArgs &a, int n_cu, hipStream_t s);
 hipError_t launch_verify_batch(const BatchVerifyArgs &a, int n_cu, hipStream_t s);

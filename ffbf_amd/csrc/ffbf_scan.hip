@@ -5,7 +5,10 @@
 // memory; one lane owns the 16 windows that start in one unit, so a wave
 // covers 62 units (992 bytes) of window starts per iteration (its top two
 // lanes load the units the windows run into) and persistent waves stride over
-// the text.  A lane loads its unit with one 128-bit non-temporal load (the
+// the text.  The LDS-prefilter instances at W 19, 24 and 32 give a lane two
+// adjacent units instead (32 windows, 126 units per wave-iteration, lane 63
+// only completes lane 62's windows), paying the first window's full hash once
+// per 32 windows.  A lane loads its unit with one 128-bit non-temporal load (the
 // next iteration's unit is loaded while the current one is hashed), looks its
 // bytes up in an LDS copy of the sbox, takes the sbox values of the following
 // bytes from the lanes above, and computes its 16 hash pairs as one full
@@ -25,9 +28,11 @@ namespace ffbf {
 
 namespace {
 
-template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false>
+template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false, int UPL = 1>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
 {
+    static_assert(UPL == 1 || (UPL == 2 && WT > 0 && !SPLIT), "two units per lane: compile-time W, pair method");
+    constexpr uint32_t IU = UPL == 2 ? kIterUnits2 : kIterUnits;
     using QG = QueueGeom<BLOCK>;
     constexpr uint32_t kSboxWords = SPLIT ? 512u : 256u;  // split: 256 (lo, hi) pairs
     extern __shared__ __align__(16) uint32_t s_pre[];
@@ -48,7 +53,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
     const uint8_t *base = a.text - lead;
     const uint64_t npos = a.n >= W ? a.n - W + 1 : 0;
     const uint64_t units = (lead + npos + 15u) >> 4;
-    const uint64_t iters = (units + kIterUnits - 1u) / kIterUnits;
+    const uint64_t iters = (units + IU - 1u) / IU;
     const uint64_t n_waves = (uint64_t)gridDim.x * (BLOCK / 64);
     uint4 *q = s_q[wv];
     uint32_t qn = 0;  // wave-uniform queue depth
@@ -56,6 +61,40 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
 
     uint4 cur = make_uint4(0u, 0u, 0u, 0u);
     uint64_t it = (uint64_t)blockIdx.x * (BLOCK / 64) + wv;
+    if constexpr (UPL == 2) {
+        // lane L owns units 2L and 2L+1 of the wave-iteration
+        uint4 cur1 = make_uint4(0u, 0u, 0u, 0u);
+        if (it < iters) {
+            const uint64_t u = it * IU + 2u * (uint64_t)lane_id();
+            cur = load_chunk(base, lead, a.n, u);
+            cur1 = load_chunk(base, lead, a.n, u + 1u);
+        }
+        for (; it < iters; it += n_waves) {
+            const uint64_t u = it * IU + 2u * (uint64_t)lane_id();
+            const uint64_t nit = it + n_waves;
+            uint4 nxt = make_uint4(0u, 0u, 0u, 0u), nxt1 = nxt;
+            if (nit < iters) {
+                nxt = load_chunk(base, lead, a.n, nit * IU + 2u * (uint64_t)lane_id());
+                nxt1 = load_chunk(base, lead, a.n, nit * IU + 2u * (uint64_t)lane_id() + 1u);
+            }
+            HashPair32<(WT > 0 ? WT : 1)> hp;
+            hp.load(cur, cur1, s_sbox);
+            uint32_t h1[kUnit], h2[kUnit];
+            hp.first(h1, h2);
+            const int64_t p0 = (int64_t)(u * 16u) - (int64_t)lead;
+            test_and_queue<LDSPRE, BLOCK, P, false, 63>(a, s_pre, h1, h2, p0, npos, a.off_base, q, qn, pend);
+            hp.second(h1, h2);
+            test_and_queue<LDSPRE, BLOCK, P, false, 63>(a, s_pre, h1, h2, p0 + 16, npos, a.off_base, q, qn, pend);
+            cur = nxt;
+            cur1 = nxt1;
+        }
+        if (qn) {
+            wave_sync();
+            drain<LDSPRE, P, false>(q, qn, a, s_pre, pend);
+        }
+        finish_pending(pend, a);
+        return;
+    }
     if constexpr (WT > 0) {
         if (it < iters) cur = load_chunk(base, lead, a.n, it * kIterUnits + (uint64_t)lane_id());
     }
@@ -92,28 +131,29 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
     finish_pending(pend, a);
 }
 
-template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false>
+template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false, int UPL = 1>
 hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
 {
     const size_t dyn = LDSPRE ? (size_t)a.pre_words * 4u : 0;
     if (dyn > 0) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_scan<WT, LDSPRE, BLOCK, P, SPLIT>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_scan<WT, LDSPRE, BLOCK, P, SPLIT, UPL>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
     }
     int per_cu = 0;
     hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<WT, LDSPRE, BLOCK, P, SPLIT>, BLOCK, dyn);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<WT, LDSPRE, BLOCK, P, SPLIT, UPL>, BLOCK, dyn);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const uint64_t W = WT ? (uint64_t)WT : a.W;
     const uint64_t npos = a.n >= W ? a.n - W + 1 : 0;
-    const uint64_t iters = ((npos + 31u) / 16u + kIterUnits - 1u) / kIterUnits;
+    const uint64_t IU = UPL == 2 ? kIterUnits2 : kIterUnits;
+    const uint64_t iters = ((npos + 31u) / 16u + IU - 1u) / IU;
     uint64_t grid = (uint64_t)n_cu * (uint64_t)per_cu;
     const uint64_t need = (iters + (BLOCK / 64) - 1) / (BLOCK / 64);
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((k_scan<WT, LDSPRE, BLOCK, P, SPLIT>), dim3((uint32_t)grid), dim3(BLOCK), dyn, s, a);
+    hipLaunchKernelGGL((k_scan<WT, LDSPRE, BLOCK, P, SPLIT, UPL>), dim3((uint32_t)grid), dim3(BLOCK), dyn, s, a);
     return hipGetLastError();
 }
 
@@ -144,9 +184,10 @@ hipError_t launch_scan_w(const ScanArgs &a, uint32_t mode, uint32_t block, int n
             if (P == 2) return launch_scan_t<WT, true, 512, 2>(a, n_cu, s);
             return launch_scan_t<WT, true, 512, 3>(a, n_cu, s);
         }
-        if (P == 1) return launch_scan_t<WT, true, 1024, 1>(a, n_cu, s);
-        if (P == 2) return launch_scan_t<WT, true, 1024, 2>(a, n_cu, s);
-        return launch_scan_t<WT, true, 1024, 3>(a, n_cu, s);
+        constexpr int U = WT > 0 ? 2 : 1;
+        if (P == 1) return launch_scan_t<WT, true, 1024, 1, false, U>(a, n_cu, s);
+        if (P == 2) return launch_scan_t<WT, true, 1024, 2, false, U>(a, n_cu, s);
+        return launch_scan_t<WT, true, 1024, 3, false, U>(a, n_cu, s);
     }
     return launch_scan_t<WT, false, 256, 1>(a, n_cu, s);
 }

@@ -83,23 +83,25 @@ struct Chunks {
 // gets lane L+1's values, L+2's on the second rotation): one LDS lookup per
 // text byte.  Lanes kIterUnits..63 get wrapped-around values and own no
 // windows.
-template <int W>
-static __device__ __forceinline__ void hash_unit(const uint4 own, const uint32_t *sb, uint32_t (&h1)[kUnit],
-                                                 uint32_t (&h2)[kUnit])
+// sbox values of the 16 bytes of one unit, from the LDS table.
+template <int N>
+static __device__ __forceinline__ void sbox_unit(const uint4 own, const uint32_t *sb, uint32_t (&s)[N], int at)
 {
-    constexpr int NB = Chunks<W>::NB;
-    uint32_t s[Chunks<W>::NCH * 16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int d = j >> 2;
         const uint32_t w = d == 0 ? own.x : d == 1 ? own.y : d == 2 ? own.z : own.w;
-        s[j] = sb[(w >> (8 * (j & 3))) & 0xFFu];
+        s[at + j] = sb[(w >> (8 * (j & 3))) & 0xFFu];
     }
-#pragma unroll
-    for (int j = 16; j < NB; j++) s[j] = lane_rol1(s[j - 16]);
-    // first window as a sum of independent rotated terms (pairs fuse into
-    // three-input XORs), the other 15 by rolling
-    uint32_t x1 = s[W - 1], x2 = s[W - 1];
+}
+
+// Hash pair of the window whose bytes' sbox values are s[0..W): a sum of
+// independent rotated terms (pairs fuse into three-input XORs).
+template <int W, int N>
+static __device__ __forceinline__ void first_window(const uint32_t (&s)[N], uint32_t &x1, uint32_t &x2)
+{
+    x1 = s[W - 1];
+    x2 = s[W - 1];
 #pragma unroll
     for (int j = 0; j + 1 < W - 1; j += 2) {
         x1 = xor3(x1, rotl(s[j], (uint32_t)((W - 1 - j) & 31)), rotl(s[j + 1], (uint32_t)((W - 2 - j) & 31)));
@@ -109,16 +111,72 @@ static __device__ __forceinline__ void hash_unit(const uint4 own, const uint32_t
         x1 ^= rotl(s[W - 2], 1u);
         x2 ^= rotl(s[W - 2], 3u);
     }
+}
+
+// Windows b0+1 .. b0+15 from window b0's pair by rolling (hash_rot_sbox_2.h:34-51):
+// slot i of h1/h2 gets window b0+i (slot 0 is window b0 itself).
+template <int W, int N>
+static __device__ __forceinline__ void roll16(const uint32_t (&s)[N], int b0, uint32_t &x1, uint32_t &x2,
+                                              uint32_t (&h1)[kUnit], uint32_t (&h2)[kUnit])
+{
     h1[0] = x1;
     h2[0] = x2;
 #pragma unroll
-    for (int b = 1; b < kUnit; b++) {
+    for (int i = 1; i < kUnit; i++) {
+        const int b = b0 + i;
         x1 = xor3(rotl(x1, 1), s[b + W - 1], rotl(s[b - 1], (uint32_t)W));
         x2 = xor3(rotl(x2, 3), s[b + W - 1], rotl(s[b - 1], (uint32_t)(3 * W)));
-        h1[b] = x1;
-        h2[b] = x2;
+        h1[i] = x1;
+        h2[i] = x2;
     }
 }
+
+template <int W>
+static __device__ __forceinline__ void hash_unit(const uint4 own, const uint32_t *sb, uint32_t (&h1)[kUnit],
+                                                 uint32_t (&h2)[kUnit])
+{
+    constexpr int NB = Chunks<W>::NB;
+    uint32_t s[Chunks<W>::NCH * 16];
+    sbox_unit(own, sb, s, 0);
+#pragma unroll
+    for (int j = 16; j < NB; j++) s[j] = lane_rol1(s[j - 16]);
+    uint32_t x1, x2;
+    first_window<W>(s, x1, x2);
+    roll16<W>(s, 0, x1, x2, h1, h2);
+}
+
+// Two units per lane: the sbox values of the lane's own 32 bytes and of the
+// W-1 bytes after them (the lane above's, by one DPP rotation each), then the
+// 32 hash pairs as one full window and 31 slides, handed out 16 at a time.
+// The first window is paid once per 32 windows instead of once per 16.
+template <int W>
+struct HashPair32 {
+    static_assert(W >= 1 && W <= 33, "the lane above supplies at most 32 bytes");
+    static constexpr int NB = 32 + W - 1;
+    uint32_t s[NB];
+    uint32_t x1 = 0, x2 = 0;
+
+    __device__ __forceinline__ void load(const uint4 u0, const uint4 u1, const uint32_t *sb)
+    {
+        sbox_unit(u0, sb, s, 0);
+        sbox_unit(u1, sb, s, 16);
+#pragma unroll
+        for (int j = 32; j < NB; j++) s[j] = lane_rol1(s[j - 32]);
+    }
+    // windows 0..15
+    __device__ __forceinline__ void first(uint32_t (&h1)[kUnit], uint32_t (&h2)[kUnit])
+    {
+        first_window<W>(s, x1, x2);
+        roll16<W>(s, 0, x1, x2, h1, h2);
+    }
+    // windows 16..31
+    __device__ __forceinline__ void second(uint32_t (&h1)[kUnit], uint32_t (&h2)[kUnit])
+    {
+        x1 = xor3(rotl(x1, 1), s[16 + W - 1], rotl(s[15], (uint32_t)W));
+        x2 = xor3(rotl(x2, 3), s[16 + W - 1], rotl(s[15], (uint32_t)(3 * W)));
+        roll16<W>(s, 16, x1, x2, h1, h2);
+    }
+};
 
 // Run-time W (1..64): bytes fetched one at a time.
 static __device__ __forceinline__ uint32_t byte_at(const uint8_t *base, uint64_t lead, uint64_t n, uint64_t q)
@@ -435,7 +493,7 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
 // survivors onto the wave's LDS queue, draining it 64 at a time.  `tag` is
 // added to the recorded offsets (the offset of a text piece, or the batched
 // scan's text id in the top bits).  Called by the whole wave.
-template <bool LDSPRE, int BLOCK, int P, bool SPLIT = false>
+template <bool LDSPRE, int BLOCK, int P, bool SPLIT = false, int OWN_LANES = (int)kIterUnits>
 static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const uint32_t *s_pre, const uint32_t (&h1)[kUnit],
                                                       const uint32_t (&h2)[kUnit], int64_t p0, uint64_t npos,
                                                       uint64_t tag, uint4 *q, uint32_t &qn, PendingCand &pend,
@@ -448,7 +506,7 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
     const uint32_t lo = (uint32_t)(lo64 < kUnit ? lo64 : kUnit);
     const uint32_t hi = (uint32_t)(hi64 <= 0 ? 0 : (hi64 < kUnit ? hi64 : kUnit));
     uint32_t vm = hi > lo ? ((0xFFFFu >> (kUnit - hi)) & (0xFFFFu << lo)) : 0u;
-    if (lane_id() >= (int)kIterUnits) vm = 0;  // these units belong to the next wave-iteration
+    if (lane_id() >= OWN_LANES) vm = 0;  // these units belong to the next wave-iteration
     vm &= line_ok;
     uint32_t pm = 0;
     if constexpr (LDSPRE) {

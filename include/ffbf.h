@@ -267,10 +267,13 @@ typedef struct ffbf_batch_match {
 } ffbf_batch_match;
 
 typedef struct ffbf_batch_tuning {
-    uint32_t chunk_iters; /* wave-iterations (992 bytes of text) per work chunk (0 = 32) */
-    uint32_t probes;      /* LDS prefilter bits tested before queueing, 1..min(k,3) (0 = auto) */
-    uint32_t block;       /* workgroup size, 512 or 1024 (0 = 1024) */
-    uint32_t use_graph;   /* 1: replay the captured hipGraph; 0: launch stage by stage */
+    uint32_t chunk_iters;    /* wave-iterations per work chunk (0 = 32); one is 992 bytes of text, or
+                                2016 with 32 bytes per lane */
+    uint32_t probes;         /* LDS prefilter bits tested before queueing, 1..min(k,3) (0 = auto) */
+    uint32_t block;          /* workgroup size, 512 or 1024 (0 = 1024) */
+    uint32_t use_graph;      /* 1: replay the captured hipGraph; 0: launch stage by stage */
+    uint32_t bytes_per_lane; /* text bytes (window starts) per lane in the LDS-prefilter scan, 16 or 32
+                                (0 = 32); 32 applies to W 19, 24 and 32 of the pair method, others use 16 */
 } ffbf_batch_tuning;
 
 int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch **out);

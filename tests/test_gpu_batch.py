@@ -110,10 +110,11 @@ def test_batch_equals_oracle_and_per_text():
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(chunk_iters=1, probes=1, block=1024, use_graph=True),
-    dict(chunk_iters=7, probes=2, block=512, use_graph=False),
-    dict(chunk_iters=64, probes=3, block=1024, use_graph=True),
+    dict(chunk_iters=1, probes=1, block=1024, use_graph=True, bytes_per_lane=32),
+    dict(chunk_iters=7, probes=2, block=512, use_graph=False, bytes_per_lane=16),
+    dict(chunk_iters=64, probes=3, block=1024, use_graph=True, bytes_per_lane=16),
     dict(chunk_iters=1000, probes=0, block=0, use_graph=False),
+    dict(chunk_iters=3, probes=3, block=512, use_graph=True, bytes_per_lane=32),
 ])
 def test_batch_tunings_agree(tuning):
     say(f"batch tuning {tuning}")
@@ -130,6 +131,11 @@ def test_batch_tunings_agree(tuning):
         assert as_rows(bt.matches()) == base
         t = bt.tuning()
         assert t["block"] in (512, 1024)
+        # W=24, pair method: 32 bytes per lane unless 16 was asked for
+        assert t["bytes_per_lane"] == (16 if tuning.get("bytes_per_lane") == 16 else 32)
+        with pytest.raises(fa.FFBFError) as e:
+            bt.set_tuning(bytes_per_lane=8)
+        assert e.value.status == fa.EINVAL
     dev.free()
     for f in filters:
         f[0].close()

@@ -165,7 +165,8 @@ def test_batch_random(case):
     rng = np.random.default_rng(9800 + case)
     W, k, sbox, filters, texts, route = _batch_case(rng)
     tuning = dict(chunk_iters=int(rng.choice([0, 1, 5, 32, 300])), probes=int(rng.integers(0, min(3, k) + 1)),
-                  block=int(rng.choice([0, 512, 1024])), use_graph=bool(rng.random() < 0.7))
+                  block=int(rng.choice([0, 512, 1024])), use_graph=bool(rng.random() < 0.7),
+                  bytes_per_lane=int(np.random.default_rng(9900 + case).choice([0, 16, 32])))
     say(f"batch case {case}: W={W} k={k} filters={len(filters)} texts={len(texts)} "
         f"bytes={sum(t.size for t in texts)} tuning={tuning}")
     rows = []
