@@ -163,6 +163,17 @@ int choose_mode_split(ffbf_ctx *c)
     return FFBF_OK;
 }
 
+// LDS prefilter bits tested up front for a prefilter of fill `fill`: enough
+// to leave few windows queued (measured on 64 MiB-1.5 GiB texts: fill 0.04,
+// 10k patterns on a 1 Mbit fold, is fastest at 2; fill 0.32, 100k patterns,
+// at 3, or at 2 by ~2% when F's first bit is then tested before queueing).
+uint32_t auto_probes(const ffbf_ctx *c, double fill)
+{
+    const uint32_t k = c->params.k;
+    if (c->probes_req) return std::min(c->probes_req, k);
+    return std::min(fill < 0.01 ? 1u : (fill < 0.2 ? 2u : 3u), k);
+}
+
 int choose_mode(ffbf_ctx *c)
 {
     c->generation++;
@@ -204,13 +215,20 @@ int choose_mode(ffbf_ctx *c)
             // fitted to MI355X measurements: the LDS stage (hash + up to 3
             // fold bits) ~1.25; one random probe ~3.8 while its array fits
             // one XCD's L2 (265 G probes/s), ~18 beyond (55 G/s); ~0.8 per
-            // window queued past the up-front bits.
+            // window queued past the up-front bits.  A fold passing many
+            // windows tests F's first bit before queueing: every window
+            // passing the up-front bits probes F once, and only those that
+            // pass it are queued.
             constexpr double kLds = 1.25, kL2 = 3.8, kFar = 18.0, kQueue = 0.8;
             const bool f_in_l2 = c->f_words * 4ull <= (4ull << 20);
             const double far = f_in_l2 ? kL2 : kFar;
             const double f = f_fill(c);
-            const double cost_fold =
-                kLds + std::pow(ff, (double)k) * far + std::pow(ff, (double)std::min(3u, k)) * kQueue;
+            const uint32_t P = std::min(auto_probes(c, ff), 3u);
+            const double pass = std::pow(ff, (double)P);
+            const bool early = fold_passes_many(ff, P);
+            const double probed = early ? pass : std::pow(ff, (double)k);
+            const double queued = early ? pass * f : pass;
+            const double cost_fold = kLds + probed * far + queued * kQueue;
             const double cost_global = (1.0 + f + f * f) * far;
             double cost_gfold = 1e30;
             if (!f_in_l2) {
@@ -243,16 +261,10 @@ int choose_mode(ffbf_ctx *c)
     RET_IF(st);
     c->mode = mode;
     // LDS prefilter bits tested up front: enough to leave few windows queued
-    if (c->probes_req) {
+    if (c->probes_req)
         c->probes = std::min(c->probes_req, k);
-    } else if (mode == ffbf::MODE_FOLD || mode == ffbf::MODE_LDS) {
-        // measured on 64 MiB-1.5 GiB texts: fill 0.04 (10k patterns, 1 Mbit
-        // fold) is fastest at 2, fill 0.32 (100k patterns) at 3
-        c->probes = c->pre_fill < 0.01 ? 1u : (c->pre_fill < 0.2 ? 2u : 3u);
-        c->probes = std::min(c->probes, k);
-    } else {
-        c->probes = 1;
-    }
+    else
+        c->probes = (mode == ffbf::MODE_FOLD || mode == ffbf::MODE_LDS) ? auto_probes(c, c->pre_fill) : 1u;
     return FFBF_OK;
 }
 
@@ -329,6 +341,7 @@ void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
     a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
     a.lds = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
+    a.f_first = (c->mode == ffbf::MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)) ? 1u : 0u;
     a.method = ffbf::METHOD_PAIR;
     if (c->split) {
         a.method = ffbf::METHOD_SPLIT;

@@ -218,5 +218,15 @@ int enqueue_pass(ffbf_ctx *c, bool fresh);
 // filters, sbox, prefilter and its geometry, probes; not the text, staging
 // or counters.
 void filter_scan_args(const ffbf_ctx *c, ScanArgs &a);
+
+// An LDS fold of fill `fill` that lets more than 5% of windows through its P
+// up-front bits: testing F's first bit right away (one round trip per 16
+// windows of a lane) beats queueing them all (a round trip per drain of 64).
+inline bool fold_passes_many(double fill, uint32_t P)
+{
+    double pass = 1.0;
+    for (uint32_t i = 0; i < P; i++) pass *= fill;
+    return pass > 0.05;
+}
 int use_device(ffbf_ctx *c);
 }  // namespace ffbf

@@ -76,6 +76,8 @@ struct ScanArgs {
     uint32_t P;          // forward bits tested against the prefilter before queueing (1..k; 1 for global)
     uint32_t pre_exact;  // prefilter is F itself: no further F probes
     uint32_t lds;        // prefilter is copied to LDS (batched scan: per filter)
+    uint32_t f_first;    // LDS fold passing many windows: F's first forward bit is tested right
+                         // after it, before queueing (one round trip per 16 windows, not per 64 queued)
     uint64_t off_base;   // added to every recorded offset (a text scanned in pieces)
     // candidate staging (cand/cand_cap above are the raw buffer)
     uint32_t *wave_counts;  // entries kept per region slot

@@ -467,7 +467,7 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
         // only where that bit is set.  F is too big for L2 whenever a fold is
         // in use, and its fill is low, so this takes ~1 random access per
         // window instead of k
-        if (ok) {
+        if (ok && !(LDSPRE && a.f_first)) {  // with f_first the bit was tested before queueing
             const uint32_t i0 = h1 & a.mask;
             ok &= (a.F[i0 >> 5] >> (i0 & 31u)) & 1u;
         }
@@ -533,6 +533,19 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const u
         for (int b = 0; b < kUnit; b++) pm |= ((fw[b] >> (h1[b] & 31u)) & 1u) << b;
     }
     pm &= vm;
+    if constexpr (LDSPRE && !SPLIT) {
+        // a fold that passes many windows: F's first forward bit (h1) now,
+        // so that only windows passing it are queued
+        if (a.f_first && pm) {
+            uint32_t fw[kUnit];
+#pragma unroll
+            for (int b = 0; b < kUnit; b++) fw[b] = ((pm >> b) & 1u) ? a.F[(h1[b] & a.mask) >> 5] : 0xFFFFFFFFu;
+            uint32_t keep = 0;
+#pragma unroll
+            for (int b = 0; b < kUnit; b++) keep |= ((fw[b] >> (h1[b] & 31u)) & 1u) << b;
+            pm &= keep;
+        }
+    }
     const uint32_t mine = (uint32_t)__builtin_popcount(pm);
     const uint32_t incl = wave_incl_sum(mine);
     const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
