@@ -117,7 +117,10 @@ int refresh_filters(ffbf_batch *b)
     for (uint32_t f = 0; f < nf; f++) {
         const ffbf_ctx *c = b->filters[f];
         const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
-        if (lds) b->pre_words_max = std::max(b->pre_words_max, c->pre_words);
+        // an R gathered in LDS sits right after the filter
+        ScanArgs probe{};
+        filter_scan_args(c, probe);
+        if (lds) b->pre_words_max = std::max(b->pre_words_max, c->pre_words * (probe.r_lds ? 2u : 1u));
         (lds ? b->has_lds : b->has_global) = true;
     }
     if (b->has_lds != had_lds || b->has_global != had_global) drop_graph(b);

@@ -82,12 +82,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
                 qn = 0;
             }
             if (loaded != 0xFFFFFFFFu) flush_pending(pend, a);
+            __syncthreads();
+            // the previous filter's reverse bits gathered in LDS (every wave
+            // has drained) go out before its prefilter is overwritten
+            if (lds && !SPLIT && loaded != 0xFFFFFFFFu && a.r_lds) flush_lds_r(s_pre + a.pre_words, a.pre_words, a.R);
             a = wave_uniform(b.filters[c.filter]);
             __syncthreads();
             if constexpr (lds) {
                 const uint4 *src = (const uint4 *)a.pre;
                 uint4 *dst = (uint4 *)s_pre;
                 for (uint32_t i = threadIdx.x; i < a.pre_words / 4u; i += BLOCK) dst[i] = src[i];
+                if (!SPLIT && a.r_lds) zero_lds(s_pre + a.pre_words, a.pre_words);
             }
             __syncthreads();
             loaded = c.filter;
@@ -151,6 +156,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_batch(BatchArgs b)
         drain<lds, lds ? P : 1, SPLIT>(q, qn, a, s_pre, pend);
     }
     if (loaded != 0xFFFFFFFFu) finish_pending(pend, a);
+    if (lds && !SPLIT && loaded != 0xFFFFFFFFu && a.r_lds) {
+        __syncthreads();
+        flush_lds_r(s_pre + a.pre_words, a.pre_words, a.R);
+    }
 }
 
 // Exact verification of a batch: the candidate's text id selects the text

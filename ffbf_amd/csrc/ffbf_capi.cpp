@@ -322,6 +322,8 @@ int enqueue_scan_begin(ffbf_ctx *c)
     return FFBF_OK;
 }
 
+constexpr uint32_t kLdsRMaxWords = 2048;  // 2^16 bits
+
 void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
 {
     a.F = c->F.p;
@@ -342,6 +344,15 @@ void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
     a.lds = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
     a.f_first = (c->mode == ffbf::MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)) ? 1u : 0u;
+    // whole F in LDS and small: R is gathered beside it in LDS.  Below 2^17
+    // bits the candidates' atomics crowd onto a few cache lines of a global
+    // R (a 2^15-bit filter with dense candidates scanned 2.9x slower for
+    // it); at 2^17 bits and above they do not, and a larger copy would only
+    // add to each workgroup's final OR
+    a.r_lds = (c->mode == ffbf::MODE_LDS && !c->split && c->pre_words <= kLdsRMaxWords &&
+               2ull * c->pre_words <= (uint64_t)ffbf::max_fold_words(c->device))
+                  ? 1u
+                  : 0u;
     a.method = ffbf::METHOD_PAIR;
     if (c->split) {
         a.method = ffbf::METHOD_SPLIT;

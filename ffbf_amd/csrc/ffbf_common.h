@@ -243,6 +243,36 @@ __device__ __forceinline__ void feed_forward(bool hit, uint32_t h1, uint32_t h2,
     feed_forward_bits<5>(hit, k, R, [&](int j) { return rev_idx(j, h1, h2) & mask; });
 }
 
+// Feed-forward into a workgroup's LDS copy of R: LDS atomics, no merging (a
+// small R in global memory concentrates every candidate's atomics on a few
+// cache lines; in LDS they stay on the CU).
+__device__ __forceinline__ void feed_forward_lds(bool hit, uint32_t h1, uint32_t h2, uint32_t k, uint32_t mask,
+                                                 uint32_t *s_R)
+{
+    if (!hit) return;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        if (j >= (int)k) break;
+        const uint32_t idx = rev_idx(j, h1, h2) & mask;
+        atomicOr(&s_R[idx >> 5], 1u << (idx & 31u));
+    }
+}
+
+// ORs a workgroup's LDS copy of R (words words) into R; every thread of the
+// block calls it after a barrier.
+__device__ __forceinline__ void flush_lds_r(const uint32_t *s_R, uint32_t words, uint32_t *R)
+{
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
+        const uint32_t v = s_R[i];
+        if (v) atomicOr(&R[i], v);
+    }
+}
+
+__device__ __forceinline__ void zero_lds(uint32_t *s, uint32_t words)
+{
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) s[i] = 0u;
+}
+
 // The same for the split method: k (<= 11) reverse bits placed in the cache
 // or external part (setBitsInFilter_2, ffbf.cpp:205-212).
 __device__ __forceinline__ void feed_forward_split(bool hit, uint32_t h1, uint32_t h2, uint32_t k, const SplitMap &m,

@@ -78,6 +78,8 @@ struct ScanArgs {
     uint32_t lds;        // prefilter is copied to LDS (batched scan: per filter)
     uint32_t f_first;    // LDS fold passing many windows: F's first forward bit is tested right
                          // after it, before queueing (one round trip per 16 windows, not per 64 queued)
+    uint32_t r_lds;      // whole F in LDS and small: R is gathered in LDS right after F (pre_words
+                         // words) and ORed into global R when the workgroup is done with the filter
     uint64_t off_base;   // added to every recorded offset (a text scanned in pieces)
     // candidate staging (cand/cand_cap above are the raw buffer)
     uint32_t *wave_counts;  // entries kept per region slot

@@ -44,6 +44,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
         const uint4 *src = (const uint4 *)a.pre;
         uint4 *dst = (uint4 *)s_pre;
         for (uint32_t i = threadIdx.x; i < a.pre_words / 4u; i += BLOCK) dst[i] = src[i];
+        if (a.r_lds) zero_lds(s_pre + a.pre_words, a.pre_words);
     }
     __syncthreads();
 
@@ -93,6 +94,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
             drain<LDSPRE, P, false>(q, qn, a, s_pre, pend);
         }
         finish_pending(pend, a);
+        if (LDSPRE && a.r_lds) {
+            __syncthreads();
+            flush_lds_r(s_pre + a.pre_words, a.pre_words, a.R);
+        }
         return;
     }
     if constexpr (WT > 0) {
@@ -129,12 +134,16 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
         drain<LDSPRE, P, SPLIT>(q, qn, a, s_pre, pend);
     }
     finish_pending(pend, a);
+    if (LDSPRE && !SPLIT && a.r_lds) {
+        __syncthreads();
+        flush_lds_r(s_pre + a.pre_words, a.pre_words, a.R);
+    }
 }
 
 template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false, int UPL = 1>
 hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
 {
-    const size_t dyn = LDSPRE ? (size_t)a.pre_words * 4u : 0;
+    const size_t dyn = LDSPRE ? (size_t)a.pre_words * 4u * (a.r_lds ? 2u : 1u) : 0;
     if (dyn > 0) {
         hipError_t e = hipFuncSetAttribute((const void *)k_scan<WT, LDSPRE, BLOCK, P, SPLIT, UPL>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);

@@ -400,7 +400,7 @@ static __device__ __forceinline__ void stage_cand(PendingCand &pc, bool hit, uin
 // tested, then (for a fold) the k forward bits of F, then feed-forward and
 // candidate staging.  Called by the whole wave.
 template <bool LDSPRE, int P, bool SPLIT = false>
-static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const ScanArgs &a, const uint32_t *s_pre,
+static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const ScanArgs &a, uint32_t *s_pre,
                                              PendingCand &pend)
 {
     const bool act = (uint32_t)lane_id() < cnt;
@@ -484,7 +484,10 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
     }
     const bool hit = ok != 0u;
     const uint64_t off = ((uint64_t)e.y << 32) | e.x;
-    feed_forward(hit, h1, h2, a.k, a.mask, a.R);
+    if (LDSPRE && a.r_lds)
+        feed_forward_lds(hit, h1, h2, a.k, a.mask, s_pre + a.pre_words);
+    else
+        feed_forward(hit, h1, h2, a.k, a.mask, a.R);
     stage_cand(pend, hit, off, h1, h2, a);
 }
 
@@ -494,7 +497,7 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
 // added to the recorded offsets (the offset of a text piece, or the batched
 // scan's text id in the top bits).  Called by the whole wave.
 template <bool LDSPRE, int BLOCK, int P, bool SPLIT = false, int OWN_LANES = (int)kIterUnits>
-static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, const uint32_t *s_pre, const uint32_t (&h1)[kUnit],
+static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, uint32_t *s_pre, const uint32_t (&h1)[kUnit],
                                                       const uint32_t (&h2)[kUnit], int64_t p0, uint64_t npos,
                                                       uint64_t tag, uint4 *q, uint32_t &qn, PendingCand &pend,
                                                       uint32_t line_ok = 0xFFFFu)
