@@ -164,16 +164,14 @@ int choose_mode_split(ffbf_ctx *c)
 }
 
 // LDS prefilter bits tested up front for a prefilter of fill `fill`: enough
-// to leave few windows queued.  Measured on 64 MiB-1.5 GiB texts: fill 0.04
-// (10k patterns on a 1 Mbit fold, k=4) is fastest at 2; fill 0.32 (100k
-// patterns, k=4) at 2, 2-3% ahead of 3, F's first bit then being tested
-// before queueing; with k=3, folds of fill 0.18-0.9 at 3.
+// to leave few windows queued (measured on 64 MiB-1.5 GiB texts: fill 0.04,
+// 10k patterns on a 1 Mbit fold, is fastest at 2; fill 0.32, 100k patterns,
+// at 3, or at 2 by ~2% when F's first bit is then tested before queueing).
 uint32_t auto_probes(const ffbf_ctx *c, double fill)
 {
     const uint32_t k = c->params.k;
     if (c->probes_req) return std::min(c->probes_req, k);
-    const double two_below = k >= 4 ? 0.4 : 0.2;  // fill up to which 2 bits are tested
-    return std::min(fill < 0.01 ? 1u : (fill < two_below ? 2u : 3u), k);
+    return std::min(fill < 0.01 ? 1u : (fill < 0.2 ? 2u : 3u), k);
 }
 
 int choose_mode(ffbf_ctx *c)
