@@ -15,6 +15,16 @@ namespace ffbf {
 
 constexpr int kUnit = 16;
 
+// (m & x) | (~m & y) as one v_bfi_b32.  Written in asm so that the compiler
+// keeps a select tree over register arrays as selects: it would otherwise
+// see an indexed read of the array and place the array in scratch memory.
+static __device__ __forceinline__ uint32_t bit_select(uint32_t m, uint32_t x, uint32_t y)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(x), "v"(y));
+    return r;
+}
+
 // Per-wave survivor queue: capacity and the depth at which it is drained
 // (64 entries at a time, one per lane).  A slot-by-slot append adds at most 64
 // entries, so cap >= drain_at + 63; a bulk append is taken only when it fits.
@@ -560,9 +570,41 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, uint32_
         // p0 itself may be negative on a text's first unit; p0 + b is not
         // for the windows written
         const uint64_t pt = tag + (uint64_t)p0;
+        // Most lanes pass at most one window: every lane's first survivor
+        // takes its hash pair through a 4-level select on its slot number
+        // (no per-slot branches), then the slots of second and later
+        // survivors, where any lane has one.
+        {
+            const uint32_t b1 = (uint32_t)__builtin_ctz(pm | 0x10000u);  // 16 (slot 0 bits) when none
+            uint32_t s1[8], s2[8];
+            const uint32_t m3 = 0u - ((b1 >> 3) & 1u), m2 = 0u - ((b1 >> 2) & 1u), m1 = 0u - ((b1 >> 1) & 1u),
+                           m0 = 0u - (b1 & 1u);
 #pragma unroll
-        for (int b = 0; b < kUnit; b++) {
-            if ((pm >> b) & 1u) {
+            for (int i = 0; i < 8; i++) {
+                s1[i] = bit_select(m3, h1[i + 8], h1[i]);
+                s2[i] = bit_select(m3, h2[i + 8], h2[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                s1[i] = bit_select(m2, s1[i + 4], s1[i]);
+                s2[i] = bit_select(m2, s2[i + 4], s2[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                s1[i] = bit_select(m1, s1[i + 2], s1[i]);
+                s2[i] = bit_select(m1, s2[i + 2], s2[i]);
+            }
+            const uint32_t v1 = bit_select(m0, s1[1], s1[0]), v2 = bit_select(m0, s2[1], s2[0]);
+            if (pm) {
+                const uint64_t p = pt + (uint64_t)b1;
+                *dst = make_uint4((uint32_t)p, (uint32_t)(p >> 32), v1, v2);
+                ++dst;
+            }
+        }
+        const uint32_t rest = pm & (pm - 1u);
+        for (uint32_t any = wave_or(rest); any; any &= any - 1u) {
+            const uint32_t b = (uint32_t)__builtin_ctz(any);  // wave-uniform: registers indexed directly
+            if ((rest >> b) & 1u) {
                 const uint64_t p = pt + (uint64_t)b;
                 *dst = make_uint4((uint32_t)p, (uint32_t)(p >> 32), h1[b], h2[b]);
                 ++dst;
