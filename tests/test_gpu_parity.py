@@ -315,3 +315,50 @@ def test_new_pattern_set_needs_a_build():
         m.build()
         r = Oracle(19, 4, 18, sbox).run(text, b2, o2)
         assert_same(gpu_all(m, text), r, m.filter())
+
+
+@pytest.mark.parametrize("W", [19, 21])
+def test_high_pass_fold_and_small_lds_filter(W):
+    """A fold so small that nearly every window passes its up-front bits
+    (F's first bit is then tested before queueing), and a 2^13-bit filter
+    held whole in LDS with dense candidates (R gathered in LDS): both equal
+    the oracle, alone and in one batch."""
+    say(f"high-pass fold / small LDS filter W={W}")
+    sbox = fa.sbox_from_seed(90 + W)
+    b1, o1, t1 = make_case(fa.ALPHA_ASCII, W, 3000, W, 50, 400_000, 200, 91)
+    b2, o2, t2 = make_case(fa.ALPHA_DNA_GC40, W, 600, W, 40, 400_000, 200, 92)
+    r1 = Oracle(W, 3, 20, sbox).run(t1, b1, o1, threads=8)
+    r2 = Oracle(W, 3, 13, sbox).run(t2, b2, o2, threads=8)
+    assert r2["cand"].size > 1000  # dense candidates on the small filter
+    m1 = fa.Matcher(W, 3, 20, sbox)
+    m1.set_patterns(b1, o1)
+    m1.build()
+    m1.set_scan_mode(fa.MODE_FOLD, 10)  # 1 Kibit fold: nearly full
+    m2 = fa.Matcher(W, 3, 13, sbox)
+    m2.set_patterns(b2, o2)
+    m2.build()
+    m2.set_scan_mode(fa.MODE_LDS)
+    for probes in (1, 2, 3):
+        m1.set_scan_tuning(probes, 0)
+        assert_same(gpu_all(m1, t1), r1)
+    assert_same(gpu_all(m2, t2), r2)
+    texts = [t1, t2]
+    off = np.zeros(3, dtype=np.uint64)
+    off[1] = t1.size
+    off[2] = t1.size + t2.size
+    dev = fa.DeviceBuffer.from_host(np.concatenate(texts))
+    expect = sorted([(0, int(p), int(q)) for p, q in zip(r1["ids"], r1["offs"])] +
+                    [(1, int(p), int(q)) for p, q in zip(r2["ids"], r2["offs"])])
+    with fa.Batch([m1, m2]) as bt:
+        bt.set_texts(dev.ptr, off, np.array([0, 1], dtype=np.uint32))
+        for tuning in (dict(), dict(probes=1, bytes_per_lane=16), dict(probes=3, block=512)):
+            bt.set_tuning(**tuning)
+            bt.run()
+            got = bt.matches()
+            rows = list(zip(got["text_id"].tolist(), got["pattern_id"].tolist(), got["offset"].tolist()))
+            assert rows == expect, tuning
+            assert np.array_equal(m1.filter(fa.FILTER_REVERSE), r1["R"])
+            assert np.array_equal(m2.filter(fa.FILTER_REVERSE), r2["R"])
+    dev.free()
+    m1.close()
+    m2.close()
