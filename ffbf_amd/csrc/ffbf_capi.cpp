@@ -164,14 +164,17 @@ int choose_mode_split(ffbf_ctx *c)
 }
 
 // LDS prefilter bits tested up front for a prefilter of fill `fill`: enough
-// to leave few windows queued (measured on 64 MiB-1.5 GiB texts: fill 0.04,
-// 10k patterns on a 1 Mbit fold, is fastest at 2; fill 0.32, 100k patterns,
-// at 3, or at 2 by ~2% when F's first bit is then tested before queueing).
+// to leave few windows queued.  Measured on 20 MiB-1.5 GiB texts: with k=4,
+// fill 0.04 (10k patterns on a 1 Mbit fold) is fastest at 2 and fill 0.32
+// (100k patterns) at 3; with k=3 (the 64-set batch's filters, DNA with
+// tandem repeats), 3 bits win from fill ~0.09 on, whole filters in LDS and
+// folds alike (2.9% over 34 filters held whole in LDS).
 uint32_t auto_probes(const ffbf_ctx *c, double fill)
 {
     const uint32_t k = c->params.k;
     if (c->probes_req) return std::min(c->probes_req, k);
-    return std::min(fill < 0.01 ? 1u : (fill < 0.2 ? 2u : 3u), k);
+    const double three_from = k <= 3 ? 0.09 : 0.2;
+    return std::min(fill < 0.01 ? 1u : (fill < three_from ? 2u : 3u), k);
 }
 
 int choose_mode(ffbf_ctx *c)
