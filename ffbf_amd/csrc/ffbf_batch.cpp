@@ -702,7 +702,9 @@ int ffbf_batch_autotune(ffbf_batch *b, ffbf_batch_tuning *best, float *best_ms)
     const ffbf_batch_tuning start = b->tuning;
     ffbf_batch_tuning top = start;
     float top_ms = 1e30f;
-    const uint32_t chunk_opts[] = {16u, 32u, 64u, 128u};
+    // 256 was 2% ahead of 128 on the 64-set batch, 512 and up fell behind
+    // (fewer chunks than workgroups can balance); 16 never won
+    const uint32_t chunk_opts[] = {32u, 64u, 128u, 256u};
     const uint32_t block_opts[] = {1024u, 512u};
     const bool any_lds = b->pre_words_max > 0;
     // 32 bytes per lane only where the LDS-class scan can take it
