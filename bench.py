@@ -517,14 +517,15 @@ def _cpu_baseline(o, n_pat, pb, po, text, budget_s):
     t0 = time.perf_counter()
     o.scan(probe, F)
     rate1 = probe.size / (time.perf_counter() - t0)
-    n1 = int(min(text.size, max(8 << 20, rate1 * budget_s * 0.25))) & ~4095
-    nT = int(min(text.size, max(8 << 20, rate1 * threads * budget_s * 0.5))) & ~4095
+    # each sample is scanned once untimed, then 5 times timed (median)
+    n1 = int(min(text.size, max(8 << 20, rate1 * budget_s * 0.25 / 6))) & ~4095
+    nT = int(min(text.size, max(8 << 20, rate1 * threads * budget_s * 0.5 / 6))) & ~4095
     res = {}
     for label, n, th in (("1", n1, 1), ("T", nT, threads)):
         sample = np.ascontiguousarray(text[:n])
-        # the threaded scan: median of three runs; one thread: one run
+        o.scan(sample, F, threads=th)  # warm
         runs = []
-        for _ in range(3 if th > 1 else 1):
+        for _ in range(5):
             t0 = time.perf_counter()
             R, cand = o.scan(sample, F, threads=th)
             runs.append(time.perf_counter() - t0)
@@ -555,8 +556,9 @@ def _cpu_baseline(o, n_pat, pb, po, text, budget_s):
         "kind": "port",
         "sample": (f"oracle build of all {n_pat} patterns ({t_build:.2f} s) + scan of the first "
                    f"{T['n'] >> 20} MiB of the text on {threads} threads ({T['scan_gbs']:.3f} GB/s) "
-                   f"(median of 3) + pattern pass + verify, scan time scaled to the full {n_text >> 20} MiB; "
-                   f"1 thread: {res['1']['gbs']:.4f} GB/s (scan {res['1']['scan_gbs']:.3f} GB/s on "
+                   f"(median of 5 after one warm run) + pattern pass + verify, scan time scaled to the full "
+                   f"{n_text >> 20} MiB; "
+                   f"1 thread: {res['1']['gbs']:.4f} GB/s (scan {res['1']['scan_gbs']:.3f} GB/s, median of 5, on "
                    f"{res['1']['n'] >> 20} MiB); host {cpu_model}, {os.cpu_count()} logical CPUs"),
         "single_thread_value": round(res["1"]["gbs"], 4),
     }
