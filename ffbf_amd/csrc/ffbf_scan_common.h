@@ -87,12 +87,6 @@ struct Chunks {
     static_assert(NCH <= 64 - (int)kIterUnits + 1, "window longer than the lanes above can supply");
 };
 
-// Hash pairs of the 16 windows of a lane's unit (W compile-time).  The lane
-// looks up the sbox values of its own 16 bytes in LDS; those of the next
-// NB-16 bytes are the lanes above's, rotated down the wave with DPP (lane L
-// gets lane L+1's values, L+2's on the second rotation): one LDS lookup per
-// text byte.  Lanes kIterUnits..63 get wrapped-around values and own no
-// windows.
 // sbox values of the 16 bytes of one unit, from the LDS table.
 template <int N>
 static __device__ __forceinline__ void sbox_unit(const uint4 own, const uint32_t *sb, uint32_t (&s)[N], int at)
@@ -141,6 +135,12 @@ static __device__ __forceinline__ void roll16(const uint32_t (&s)[N], int b0, ui
     }
 }
 
+// Hash pairs of the 16 windows of a lane's unit (W compile-time).  The lane
+// looks up the sbox values of its own 16 bytes in LDS; those of the next
+// NB-16 bytes are the lanes above's, rotated down the wave with DPP (lane L
+// gets lane L+1's values, L+2's on the second rotation): one LDS lookup per
+// text byte.  Lanes kIterUnits..63 get wrapped-around values and own no
+// windows.
 template <int W>
 static __device__ __forceinline__ void hash_unit(const uint4 own, const uint32_t *sb, uint32_t (&h1)[kUnit],
                                                  uint32_t (&h2)[kUnit])
