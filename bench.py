@@ -660,6 +660,8 @@ def main() -> int:
                     help="only run the plain copy kernel over BYTES (HBM counter reference) and exit")
     ap.add_argument("--gather-sweep", action="store_true",
                     help="only run the random-gather microbenchmark (4-byte probes into 16 KiB..2 GiB) and exit")
+    ap.add_argument("--gather-log2", type=int, default=0, metavar="L",
+                    help="only run the random-gather microbenchmark into a 2^L-byte array (two launches) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -679,6 +681,12 @@ def main() -> int:
         for _ in range(3):
             dst.copy_from(src, n)
         print(json.dumps({"copy_calibration_bytes": n, "launches": 3}), flush=True)
+        return 0
+
+    if args.gather_log2:
+        ms, rate = fa.gather_bench(args.gather_log2, steps=16)
+        print(json.dumps({"gather_array_bytes": 1 << args.gather_log2, "ms": round(ms, 4),
+                          "probes_per_s": round(rate, 1)}), flush=True)
         return 0
 
     if args.gather_sweep:

@@ -215,7 +215,10 @@ __device__ __forceinline__ void feed_forward_bits(bool hit, uint32_t k, uint32_t
     if (!hits) return;
     const bool dense = __builtin_popcountll(hits) > 4;
     const int lane = lane_id();
-#pragma unroll
+    // a rolled loop (bit_of(j) then selects by j at run time): it runs for
+    // waves holding hits only, and its per-word merge loop keeps the
+    // compiler from flattening it anyway
+#pragma nounroll
     for (int j = 0; j < KMAX; j++) {
         if (j >= (int)k) break;
         const uint32_t idx = bit_of(j);
