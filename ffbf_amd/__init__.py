@@ -134,6 +134,7 @@ ABI = {
     "ffbf_write_survivors": (_i32, [_P, _c.c_char_p, _c.c_char_p]),
     "ffbf_write_candidate_lines": (_i32, [_P, _P, _u64, _c.c_char_p]),
     "ffbf_run_device": (_i32, [_P, _P, _u64, _pu64]),
+    "ffbf_run_device_noalloc": (_i32, [_P, _P, _u64, _P, _u64, _pu64]),
     "ffbf_pattern_pass": (_i32, [_P, _pu64]),
     "ffbf_verify": (_i32, [_P, _pu64]),
     "ffbf_get_matches": (_i32, [_P, _P, _u64, _pu64]),

@@ -229,4 +229,6 @@ inline bool fold_passes_many(double fill, uint32_t P)
     return pass > 0.05;
 }
 int use_device(ffbf_ctx *c);
+// Workgroup size of the context's scan launches (its scan mode and tuning).
+uint32_t scan_block(ffbf_ctx *c);
 }  // namespace ffbf

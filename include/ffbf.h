@@ -176,6 +176,26 @@ int ffbf_scan_device(ffbf_ctx *ctx, const uint8_t *dev_text, uint64_t n);
  * the matches are then read with ffbf_get_matches. */
 int ffbf_run_device(ffbf_ctx *ctx, const uint8_t *dev_text, uint64_t n, uint64_t *n_matches);
 
+/* ffbf_run_device without a single device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The context must have been created, given patterns and built
+ * beforehand (that is where its resident state was allocated).  The call's
+ * own buffers (scan counters, candidate staging and list, match list, sort
+ * storage) live in a fixed 32 MiB device array of the library, one copy per
+ * GPU, reused by every call on that GPU; calls on one GPU are serialised by
+ * the library (they would race on that array).  Limits of that array: at most
+ * 262,144 candidate windows and 262,144 matches per call (FFBF_ECAPACITY
+ * beyond, the text being left to ffbf_run_device), at most 2^24 patterns and
+ * 2^40 text bytes (FFBF_EINVAL).  dev_matches: device memory of cap entries;
+ * the matches land there sorted by (pattern id, offset), the same list
+ * ffbf_run_device + ffbf_get_matches give; *n_matches receives the total
+ * (FFBF_ECAPACITY if cap is smaller, nothing copied).  R and the survivor set
+ * are updated as by ffbf_run_device; the context's candidate and match
+ * getters do not see this call's results. */
+int ffbf_run_device_noalloc(ffbf_ctx *ctx, const uint8_t *dev_text, uint64_t n, ffbf_match *dev_matches, uint64_t cap,
+                            uint64_t *n_matches);
+
 /* Pattern pass against R (executePatternFiltering, greptest.cu:578-596).
  * n_survivors may be NULL. */
 int ffbf_pattern_pass(ffbf_ctx *ctx, uint64_t *n_survivors);
