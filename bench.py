@@ -475,6 +475,9 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
         "scan_probes_per_s": round(rate, 1),
         "gather_probes_per_s": round(ceiling, 1),
         "frac": round(rate / ceiling, 4) if ceiling > 0 else None,
+        # L2 counters of this scan from the committed rocprofv3 run: requests per
+        # window and per L2 channel-cycle, beside the gather kernel's
+        "counters": traffic_from_profiles("plain_l2"),
     }
 
 
@@ -790,6 +793,8 @@ def main() -> int:
         "roofline": {
             "kernel": "k_scan",
             "bound": "hbm",
+            # what holds the kernel below that roof (DESIGN: L2 counters, SQ counters)
+            "limiter": ("L2 random-request rate" if mode["scan_mode"] in (0, 3) else "VALU/LDS instruction issue"),
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
