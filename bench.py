@@ -793,7 +793,7 @@ def main() -> int:
         "roofline": {
             "kernel": "k_scan",
             "bound": "hbm",
-            # what holds the kernel below that roof (DESIGN: L2 counters, SQ counters)
+            # what holds the kernel below that roof, from its L2 and SQ counters
             "limiter": ("L2 random-request rate" if mode["scan_mode"] in (0, 3) else "VALU/LDS instruction issue"),
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
