@@ -214,13 +214,14 @@ def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0):
     }
 
 
-def serve_inputs(fa, n_requests, seed=51):
+def serve_inputs(fa, n_requests, seed=51, forced=None):
     """Serving workload: 8 resident pattern sets, n log-spaced 10k..2M
     (lengths 24-80; sets alternate DNA and printable ASCII), W=24, k=3,
     filter bits = 16 n rounded up to a power of two (the multi-filter rule);
     requests with log-normal sizes (median 512 KiB, sigma 1.5, clipped to
     1 KiB-256 MiB), each routed to a seeded set and drawn over that set's
-    alphabet, one planted occurrence per started MiB."""
+    alphabet, one planted occurrence per started MiB.  forced: {request
+    index: size} overriding drawn sizes (the range's edges)."""
     import numpy as np
 
     W, k = 24, 3
@@ -232,6 +233,8 @@ def serve_inputs(fa, n_requests, seed=51):
         pb, po = fa.gen_patterns(alpha, seed * 100 + f, n, 24, 80)
         sets.append((pb, po, next_pow2(16 * n).bit_length() - 1, alpha))
     sizes = fa.lognormal_sizes(seed, n_requests, 512 * 1024.0, 1.5, 1024, 256 << 20)
+    for i, sz in (forced or {}).items():
+        sizes[i] = sz
     route = [int(fa.gen_u64(seed, 23, i) % 8) for i in range(n_requests)]
     # request texts live in page-locked host memory, as a serving frontend's
     # receive buffers would

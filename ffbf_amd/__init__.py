@@ -39,7 +39,7 @@ if not os.path.exists(LIB_PATH):
 _lib = ctypes.CDLL(LIB_PATH)
 
 OK = 0
-EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY = -1, -2, -3, -4, -5
+EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY, ECANCELED = -1, -2, -3, -4, -5, -6
 FILTER_FORWARD, FILTER_REVERSE = 0, 1
 MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS, MODE_GFOLD = -1, 0, 1, 2, 3
 ALPHA_ASCII, ALPHA_DNA_GC40 = 0, 1
@@ -173,6 +173,7 @@ ABI = {
     "ffbf_server_destroy": (None, [_P]),
     "ffbf_server_submit": (_i32, [_P, _u32, _P, _u64, _pu64]),
     "ffbf_server_submit_many": (_i32, [_P, _u32, _P, _P, _P, _P]),
+    "ffbf_server_cancel": (_i32, [_P, _u64]),
     "ffbf_server_poll": (_i32, [_P, _u64, _c.POINTER(_c.c_int)]),
     "ffbf_server_wait": (_i32, [_P, _u64, _c.POINTER(_P), _pu64, _c.POINTER(_c.c_double)]),
     "ffbf_server_get_stats": (_i32, [_P, _P]),
@@ -706,6 +707,15 @@ class Server:
         d = _c.c_int(0)
         _check(_lib.ffbf_server_poll(self._h, rid, _c.byref(d)))
         return bool(d.value)
+
+    def cancel(self, rid: int) -> bool:
+        """True if the request will complete cancelled, False if it had
+        completed already (wait() then returns its matches)."""
+        st = _lib.ffbf_server_cancel(self._h, rid)
+        if st == ESTATE:
+            return False
+        _check(st)
+        return True
 
     def wait(self, rid: int):
         out = _c.c_void_p(0)

@@ -49,6 +49,7 @@ extern "C" {
 #define FFBF_ENOMEM (-3)    /* device or host allocation failed */
 #define FFBF_ESTATE (-4)    /* call out of order (e.g. scan before build) */
 #define FFBF_ECAPACITY (-5) /* caller buffer too small; required size reported */
+#define FFBF_ECANCELED (-6) /* request cancelled before it completed */
 
 #define FFBF_FILTER_FORWARD 0 /* F: built from patterns */
 #define FFBF_FILTER_REVERSE 1 /* R: fed forward by the last scan */
@@ -324,7 +325,9 @@ int ffbf_batch_timings(ffbf_batch *batch, float *run_ms, float *scan_ms);
  * index), and one worker per slot: a free slot admits, in arrival order,
  * every pending request that fits its text buffer and the HBM budget, stages
  * the texts, scans them as one batch on its own stream and completes them;
- * slots overlap on the device.  A request's matches equal what
+ * slots overlap on the device.  A request longer than a slot's buffer is
+ * scanned in pieces of that size overlapping by the filter's longest pattern
+ * minus one byte, each match reported once.  A request's matches equal what
  * ffbf_match_text gives for its text alone.  The contexts passed in are only
  * read at creation. */
 typedef struct ffbf_server ffbf_server;
@@ -346,12 +349,19 @@ int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_
  * last ffbf_server_wait. */
 void ffbf_server_destroy(ffbf_server *server);
 /* Queues a scan of text[0..n) (host memory, borrowed until the request
- * completes) against filter `filter`.  FFBF_ECAPACITY if the request can
- * never be admitted (larger than a slot or the budget). */
+ * completes) against filter `filter`.  FFBF_ECAPACITY if a piece of it can
+ * never be admitted (a slot's buffer, with its candidate reserve, larger
+ * than the HBM budget); FFBF_EINVAL if it needs pieces and the filter's
+ * longest pattern exceeds half a slot. */
 int ffbf_server_submit(ffbf_server *server, uint32_t filter, const uint8_t *text, uint64_t n, uint64_t *request_id);
 /* ffbf_server_submit for n requests at once (stops at the first error). */
 int ffbf_server_submit_many(ffbf_server *server, uint32_t n, const uint32_t *filters, const uint8_t *const *texts,
                             const uint64_t *sizes, uint64_t *request_ids);
+/* Cancels a request: its pieces not yet on the device are dropped and it
+ * completes with FFBF_ECANCELED (at once, or when its pieces already being
+ * scanned are done; its text is borrowed until then).  FFBF_ESTATE if it had
+ * completed already (ffbf_server_wait still returns its result). */
+int ffbf_server_cancel(ffbf_server *server, uint64_t request_id);
 int ffbf_server_poll(ffbf_server *server, uint64_t request_id, int *done);
 /* Blocks until the request completes; *matches (sorted by pattern id, then
  * offset) is allocated by the library (ffbf_free_matches); latency_ms is

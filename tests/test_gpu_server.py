@@ -128,11 +128,37 @@ def test_server_admission_limits():
     say("server admission")
     sbox = fa.sbox_from_seed(92)
     filters = make_filters(sbox)[:2]
-    with fa.Server([f[0] for f in filters], hbm_budget_bytes=4 << 20, slots=2, max_batch_requests=3) as srv:
-        big = np.zeros(3 << 20, dtype=np.uint8)  # larger than one slot (2 MiB)
+    # a slot's buffer (with its candidate reserve) larger than the budget:
+    # requests needing a whole slot can never be admitted
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=1 << 20, slots=1, max_batch_bytes=2 << 20) as tight:
         with pytest.raises(fa.FFBFError) as e:
-            srv.submit(0, big)
+            tight.submit(0, np.zeros(3 << 20, dtype=np.uint8))
         assert e.value.status == fa.ECAPACITY
+        assert tight.stats()["rejected"] == 1
+        small = make_requests(filters, 1, 6, 100_000)[0]
+        got, _ = tight.wait(tight.submit(*small))
+        assert np.array_equal(got["offset"], alone(filters, *small)["offset"])
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=4 << 20, slots=2, max_batch_requests=3) as srv:
+        # larger than one slot (2 MiB): scanned in two overlapping pieces
+        big = fa.gen_bytes(filters[0][4], 77, 3 << 20)
+        fa.plant(78, 300, big, filters[0][1], filters[0][2])
+        # plants around the piece border: the second piece owns the window
+        # starts from 2 MiB - (longest pattern - 1) on; one plant starts just
+        # below that and ends at the first piece's last byte
+        b, off = filters[0][1], filters[0][2]
+        lens = np.diff(off)
+        border = (2 << 20) - (int(lens.max()) - 1)
+        at_list = [border - 150, border - 75, border - 1, border + 72, border + 150]
+        longest = int(np.argmax(lens))
+        for j, at in zip([3, 4, longest, 5, 6], at_list):
+            p = b[int(off[j]):int(off[j + 1])]
+            big[at:at + p.size] = p
+        expect_big = alone(filters, 0, big)
+        got, _ = srv.wait(srv.submit(0, big))
+        for at in at_list:
+            assert at in set(expect_big["offset"].tolist()), at
+        assert np.array_equal(got["pattern_id"], expect_big["pattern_id"])
+        assert np.array_equal(got["offset"], expect_big["offset"])
         with pytest.raises(fa.FFBFError):
             srv.submit(5, b"x" * 100)  # no such filter
         # many requests that only fit a few at a time: all complete, in budget
@@ -142,7 +168,7 @@ def test_server_admission_limits():
         for i, rid in enumerate(ids):
             got, _ = srv.wait(rid)
             assert np.array_equal(got["offset"], expect[i]["offset"]), i
-        assert srv.stats()["rejected"] == 1
+        assert srv.stats()["rejected"] == 0
         # two waits on one request: exactly one gets the matches, the other
         # an error (already waited for, or already released)
         f0, t0 = reqs[0]
