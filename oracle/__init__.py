@@ -6,9 +6,11 @@ cites the reference lines it follows).  Only tests/, the smoke check and
 bench.py's cpu_baseline leg may import this package, and only as the checker
 or the timed CPU baseline, never as part of the product path.
 
-Parity status of each function (how it is pinned without running the
-reference, which does not build here):
-  hash_window       closed form XOR of rotations, the W=32 all-rotations
+Parity status of each function (how it is pinned; of the reference only its
+rolling-hash headers build here, see oracle/ref.mk):
+  hash_window       the reference's own hash_rot_sbox_2.h compiled unmodified
+                    (oracle/_ref, every window of random texts, W 1..64),
+                    closed form XOR of rotations, the W=32 all-rotations
                     parity identity, W=1/W=2 hand values from the reference sbox
   hash_rolling      equals hash_window on every window; tail-only dependence
                     (the reference's validate.rb invariant)
@@ -22,7 +24,9 @@ reference, which does not build here):
                     (newline runs, a line without '\n', a candidate on a line
                     break, several candidates in one line)
   split method (SplitOracle, the CPU binary's filter)
-    hash            closed form XOR of 64-bit rotations (golden vectors from
+    hash            the reference's own hash_rot_sbox_pre_2.h compiled
+                    unmodified (oracle/_ref, W 1..64, bytes below 0x80),
+                    closed form XOR of 64-bit rotations (golden vectors from
                     the reference's 64-bit sbox table), the W=64 all-rotations
                     parity identity, W=1 = the sbox entry, 0x80..0xFF hashing
                     as 0x00
