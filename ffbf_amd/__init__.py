@@ -141,6 +141,8 @@ ABI = {
     "ffbf_match_text": (_i32, [_P, _P, _u64, _c.POINTER(_P), _pu64]),
     "ffbf_free_matches": (None, [_P]),
     "ffbf_get_filter": (_i32, [_P, _i32, _P, _u64]),
+    "ffbf_query_line": (_i32, [_P, _P, _u64, _c.POINTER(_c.c_int)]),
+    "ffbf_query_lines": (_i32, [_P, _P, _P, _u64, _P]),
     "ffbf_get_candidates": (_i32, [_P, _P, _u64, _pu64]),
     "ffbf_get_survivors": (_i32, [_P, _P, _u64, _pu64]),
     "ffbf_filter_save": (_i32, [_P, _P, _u64, _pu64]),
@@ -511,6 +513,22 @@ class Matcher:
         finally:
             _lib.ffbf_free_matches(out)
         return Result(arr)
+
+    # -- line queries (Filter::operator[])
+    def query_line(self, line) -> bool:
+        t = np.frombuffer(bytes(line), dtype=np.uint8) if isinstance(line, (bytes, bytearray)) else np.ascontiguousarray(line)
+        h = _c.c_int(0)
+        _check(_lib.ffbf_query_line(self._h, _ptr(t), t.size, _c.byref(h)))
+        return bool(h.value)
+
+    def query_lines(self, line_bytes: np.ndarray, line_off: np.ndarray) -> np.ndarray:
+        """uint8 0/1 per line: line i is line_bytes[line_off[i]:line_off[i+1]]."""
+        line_bytes = np.ascontiguousarray(line_bytes, dtype=np.uint8)
+        line_off = np.ascontiguousarray(line_off, dtype=np.uint64)
+        n = line_off.size - 1
+        hits = np.zeros(max(n, 1), dtype=np.uint8)
+        _check(_lib.ffbf_query_lines(self._h, _ptr(line_bytes), _ptr(line_off), n, _ptr(hits)))
+        return hits[:n]
 
     # -- results
     def matches(self) -> np.ndarray:

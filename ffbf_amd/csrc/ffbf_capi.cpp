@@ -689,6 +689,9 @@ void ffbf_destroy(ffbf_ctx *c)
     c->matches.release();
     c->sort_keys.release();
     c->sort_temp.release();
+    c->q_bytes.release();
+    c->q_hits.release();
+    c->q_off.release();
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     for (int i = 0; i < EV_N; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -932,6 +935,52 @@ int ffbf_match_text(ffbf_ctx *c, const uint8_t *text, uint64_t n, ffbf_match **o
 }
 
 void ffbf_free_matches(ffbf_match *m) { std::free(m); }
+
+int ffbf_query_lines(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets, uint64_t n_lines, uint8_t *hits)
+{
+    RET_IF(check_ctx(c));
+    if (!c->built) return fail(FFBF_ESTATE, "query before build (or filter load)");
+    if (n_lines == 0) return FFBF_OK;
+    if (!offsets || !hits) return fail(FFBF_EINVAL, "null offsets or hits");
+    if (offsets[0] != 0) return fail(FFBF_EINVAL, "offsets[0] must be 0");
+    for (uint64_t i = 0; i < n_lines; i++)
+        if (offsets[i + 1] < offsets[i]) return fail(FFBF_EINVAL, "offsets must be non-decreasing");
+    const uint64_t total = offsets[n_lines];
+    if (total && !bytes) return fail(FFBF_EINVAL, "null line bytes");
+    RET_IF(c->q_bytes.ensure(std::max<uint64_t>(total, 1), "query lines"));
+    RET_IF(c->q_off.ensure(n_lines + 1, "query offsets"));
+    RET_IF(c->q_hits.ensure(n_lines, "query answers"));
+    if (total) HIP_TRY(hipMemcpyAsync(c->q_bytes.p, bytes, total, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->q_off.p, offsets, (n_lines + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    ffbf::QueryArgs a{};
+    a.bytes = c->q_bytes.p;
+    a.off = c->q_off.p;
+    a.n_lines = n_lines;
+    a.F = c->F.p;
+    a.sbox = c->sbox.p;
+    a.W = c->params.window;
+    a.k = c->params.k;
+    a.mask = c->mask;
+    a.method = c->split ? ffbf::METHOD_SPLIT : ffbf::METHOD_PAIR;
+    a.split = c->smap;
+    a.hits = c->q_hits.p;
+    HIP_TRY(ffbf::launch_query_lines(a, c->n_cu, c->stream));
+    HIP_TRY(hipMemcpyAsync(hits, c->q_hits.p, n_lines, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FFBF_OK;
+}
+
+int ffbf_query_line(ffbf_ctx *c, const uint8_t *line, uint64_t len, int *hit)
+{
+    if (!hit) return fail(FFBF_EINVAL, "null output");
+    *hit = 0;
+    if (!line && len) return fail(FFBF_EINVAL, "null line");
+    const uint64_t off[2] = {0, len};
+    uint8_t h = 0;
+    RET_IF(ffbf_query_lines(c, line, off, 1, &h));
+    *hit = h;
+    return FFBF_OK;
+}
 
 int ffbf_get_filter(ffbf_ctx *c, int which, uint32_t *out, uint64_t n_words)
 {

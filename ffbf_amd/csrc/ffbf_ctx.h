@@ -176,6 +176,10 @@ struct ffbf_ctx {
     hipEvent_t ev[ffbf::EV_N] = {};
     bool ev_recorded[ffbf::EV_N] = {};
 
+    // line queries (ffbf_query_lines): the lines, their offsets and answers
+    ffbf::DevBuf<uint8_t> q_bytes, q_hits;
+    ffbf::DevBuf<uint64_t> q_off;
+
     // text ingest from host memory and files (ffbf_ingest.cpp)
     hipStream_t copy_stream = nullptr;
     uint8_t *staging[4] = {};         // pinned pieces of the text on their way to the device

@@ -182,6 +182,19 @@ struct PassArgs {
     SplitMap split;
 };
 
+// Line queries (Filter::operator[]): line i is bytes[off[i] .. off[i+1]).
+struct QueryArgs {
+    const uint8_t *bytes;
+    const uint64_t *off;
+    uint64_t n_lines;
+    const uint32_t *F;
+    const uint32_t *sbox;  // split method: 256 (lo, hi) pairs, entries 0x80..0xFF = entry 0
+    uint32_t W, k, mask;
+    uint32_t method;  // HashMethod
+    SplitMap split;
+    uint8_t *hits;    // per line: 1 if some window has its k forward bits set in F
+};
+
 // A run of u32 words to clear (a few thousand words each, so that many
 // workgroups share a large buffer).
 struct ZeroSpan {
@@ -217,6 +230,7 @@ hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
                                unsigned long long *counters, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
+hipError_t launch_query_lines(const QueryArgs &a, int n_cu, hipStream_t s);
 
 // units_per_lane 2 (32 window starts per lane) applies to compile-time W of
 // the pair method; chunks of the LDS class are then counted in wave-iterations

@@ -278,6 +278,57 @@ __global__ __launch_bounds__(256) void k_verify(VerifyArgs a)
     }
 }
 
+// Filter::operator[] (ffbf.cpp:285-308) for many lines: one wave per line,
+// lanes over the line's window starts 64 at a time, each hashing its window
+// from the W bytes (the line is hashed from a reset and never restarted
+// inside it, so every window's hash is its W bytes' alone) and testing its k
+// forward bits in F.  The wave stops at the first window that hits.
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_query_lines(QueryArgs a)
+{
+    __shared__ uint32_t s_sb[512];
+    constexpr uint32_t words = SPLIT ? 512u : 256u;
+    for (uint32_t i = threadIdx.x; i < words; i += 256u) s_sb[i] = a.sbox[i];
+    __syncthreads();
+    const uint64_t n_waves = (uint64_t)gridDim.x * 4u;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t l = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6); l < a.n_lines; l += n_waves) {
+        const uint64_t p0 = a.off[l], len = a.off[l + 1] - p0;
+        bool any = false;
+        if (len >= a.W) {
+            const uint64_t nwin = len - a.W + 1u;
+            for (uint64_t base = 0; base < nwin && !any; base += 64u) {
+                const uint64_t o = base + lane;
+                bool hit = false;
+                if (o < nwin) {
+                    const uint8_t *w = a.bytes + p0 + o;
+                    uint32_t h1 = 0, h2 = 0;
+                    for (uint32_t j = 0; j < a.W; j++) {
+                        const uint32_t c = w[j];
+                        if constexpr (SPLIT) {
+                            uint32_t rl, rh;
+                            rol64(h1, h2, 1u, rl, rh);
+                            h1 = rl ^ s_sb[2u * c];
+                            h2 = rh ^ s_sb[2u * c + 1u];
+                        } else {
+                            h1 = rotl(h1, 1) ^ s_sb[c];
+                            h2 = rotl(h2, 3) ^ s_sb[c];
+                        }
+                    }
+                    hit = true;
+                    for (uint32_t j = 0; j < a.k && hit; j++) {
+                        const uint32_t x = SPLIT ? split_place((int)j, split_fwd_fn((int)j, h1, h2), a.split)
+                                                 : fwd_idx((int)j, h1, h2) & a.mask;
+                        hit = ((a.F[x >> 5] >> (x & 31u)) & 1u) != 0u;
+                    }
+                }
+                any = __ballot(hit) != 0;
+            }
+        }
+        if (lane == 0) a.hits[l] = any ? 1u : 0u;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s)
@@ -476,6 +527,19 @@ hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cand_gather, dim3((uint32_t)(n_cu * 2)), dim3(256), 0, s, raw, offsets, n_regions, region,
                        spill_base, out, out_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_lines(const QueryArgs &a, int n_cu, hipStream_t s)
+{
+    if (a.n_lines == 0) return hipSuccess;
+    uint64_t blocks = (a.n_lines + 3u) / 4u;
+    const uint64_t cap = (uint64_t)n_cu * 8u;
+    if (blocks > cap) blocks = cap;
+    if (a.method == METHOD_SPLIT)
+        hipLaunchKernelGGL(k_query_lines<true>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_query_lines<false>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@
  *   printpatterns             src/gpgpu/greptest.cu:400-426   -> ffbf_write_survivors
  *   Filter::save/loadFilterToFile  src/rabin-karp/ffbf.cpp:487-527 -> ffbf_filter_save / ffbf_filter_load
  *   Filter<hash_rot_sbox_pre_2,19> src/rabin-karp/ffbf.cpp:88-185  -> ffbf_create_split
+ *   Filter::operator[]        src/rabin-karp/ffbf.cpp:285-308 -> ffbf_query_line / ffbf_query_lines
  *
  * Conventions: every call returns FFBF_OK (0) or a negative status and never
  * exits or throws; ffbf_last_error() gives the calling thread's last message.
@@ -227,6 +228,19 @@ int ffbf_write_survivors(ffbf_ctx *ctx, const char *patterns_path, const char *i
  * '\n' ends at the end of the text.  text/n: the caller's host copy of the
  * scanned text, or NULL/0 to read it back from the device. */
 int ffbf_write_candidate_lines(ffbf_ctx *ctx, const uint8_t *text, uint64_t n, const char *out_path);
+
+/* Line query (Filter::operator[], ffbf.cpp:285-308): *hit = 1 iff the line
+ * is at least W bytes long and one of its windows -- hashed from a reset
+ * over the line, not restarted inside it (for the split method bytes >= 0x80
+ * hash as 0, ffbf.cpp:124-133) -- has all k forward bits set in F.  Nothing is
+ * fed forward; R and the results of the last scan are untouched.  Needs a
+ * built (or loaded) filter. */
+int ffbf_query_line(ffbf_ctx *ctx, const uint8_t *line, uint64_t len, int *hit);
+
+/* The same for n_lines lines at once, line i being bytes[offsets[i] ..
+ * offsets[i+1]) (offsets has n_lines+1 entries, offsets[0] = 0); hits[i]
+ * receives 0 or 1.  One launch for all lines (one wave per line). */
+int ffbf_query_lines(ffbf_ctx *ctx, const uint8_t *bytes, const uint64_t *offsets, uint64_t n_lines, uint8_t *hits);
 
 /* Filter dump (bvDump, greptest.cu:341-351): n_words must be >= 2^log2_bits/32. */
 int ffbf_get_filter(ffbf_ctx *ctx, int which, uint32_t *out, uint64_t n_words);

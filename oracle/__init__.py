@@ -19,6 +19,11 @@ rolling-hash headers build here, see oracle/ref.mk):
                     order independence, popcount bounds
   verify            independent naive matcher (brute_matches) and planted
                     occurrences
+  query_lines       a line's answer equals "a scan of the line as a text has
+                    a candidate" (scan is pinned above), lines shorter than W
+                    are false, hand lines holding a pattern; split method:
+                    the same over newline-free lines, and a line with a '\n'
+                    inside is hashed through it (hand-computed)
   save/load deltas  round trip and hand-built vectors
   candidate_lines / survivor_files   hand-built texts and pattern sets
                     (newline runs, a line without '\n', a candidate on a line
@@ -94,6 +99,8 @@ _sig = {
     "orc_split_scan_mt": (_u64, [_P, _P, _u64, _P, _P, _P, _u64, ctypes.c_int]),
     "orc_split_pattern_pass": (_u64, [_P, _P, _P, _u64, _P, _P]),
     "orc_split_verify": (_u64, [_P, _P, _u64, _P, _P, _u64, _P, _u64, _P, _P, _P, _u64]),
+    "orc_query_lines": (_u64, [_P, _P, _P, _u64, _P, _P]),
+    "orc_split_query_lines": (_u64, [_P, _P, _P, _u64, _P, _P]),
 }
 for _n, (_r, _a) in _sig.items():
     _f = getattr(_lib, _n)
@@ -180,6 +187,14 @@ class Oracle:
             if m <= cap:
                 return ids[:m], offs[:m]
             cap = int(m)
+
+    def query_lines(self, line_bytes: np.ndarray, line_off: np.ndarray, F: np.ndarray) -> np.ndarray:
+        """Filter::operator[] per line (ffbf.cpp:285-308): uint8 0/1 per line."""
+        n = line_off.size - 1
+        hits = np.zeros(max(n, 1), dtype=np.uint8)
+        fn = _lib.orc_split_query_lines if isinstance(self, SplitOracle) else _lib.orc_query_lines
+        fn(self._pp, _ptr(line_bytes), _ptr(line_off), n, _ptr(F), _ptr(hits))
+        return hits[:n]
 
     def run(self, text, pat_bytes, pat_off, threads: int = 1) -> dict:
         F = self.build(pat_bytes, pat_off)

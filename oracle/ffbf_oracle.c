@@ -678,3 +678,61 @@ uint64_t orc_split_verify(const orc_split_params *p, const uint8_t *text, uint64
     return verify_core(p, split_hash_cb, p->window, text, n, bytes, off, n_patterns, cand, n_cand, surv_bitmap,
                        out_id, out_off, cap);
 }
+
+/* Filter::operator[] (ffbf.cpp:285-308) over many lines: a line shorter
+ * than W is false; otherwise the hash runs over the line from a reset, byte
+ * by byte (the line is not restarted inside: operator[] takes one line), and
+ * the line is true as soon as one window has its k forward bits set in F
+ * (checkInFilter, :214-239).  Nothing is fed forward.  Line i is
+ * bytes[off[i] .. off[i+1]); hits[i] gets 0 or 1.  Returns the number of
+ * true lines.  The pair method: the GPU arithmetic's window hash and forward
+ * family on a 2^log2_bits-bit F. */
+uint64_t orc_query_lines(const orc_params *p, const uint8_t *bytes, const uint64_t *off, uint64_t n_lines,
+                         const uint32_t *F, uint8_t *hits)
+{
+    const uint32_t W = p->window;
+    uint64_t n_hit = 0;
+    for (uint64_t i = 0; i < n_lines; i++) {
+        const uint64_t len = off[i + 1] - off[i];
+        const uint8_t *line = bytes + off[i];
+        int hit = 0;
+        for (uint64_t o = 0; len >= W && o + W <= len && !hit; o++) {
+            uint32_t h1, h2, idx[5];
+            orc_hash_window(p, line + o, &h1, &h2);
+            orc_fwd_indices(p, h1, h2, idx);
+            hit = 1;
+            for (uint32_t j = 0; j < p->k && hit; j++) hit = test_bit(F, idx[j]);
+        }
+        hits[i] = (uint8_t)hit;
+        n_hit += (uint64_t)hit;
+    }
+    return n_hit;
+}
+
+/* The same for the CPU binary's split filter: its ring-buffer update from a
+ * reset (bytes >= 0x80 hashed as 0), tested once the hash is full. */
+uint64_t orc_split_query_lines(const orc_split_params *p, const uint8_t *bytes, const uint64_t *off,
+                               uint64_t n_lines, const uint32_t *F, uint8_t *hits)
+{
+    uint64_t n_hit = 0;
+    for (uint64_t i = 0; i < n_lines; i++) {
+        const uint64_t len = off[i + 1] - off[i];
+        const uint8_t *line = bytes + off[i];
+        int hit = 0;
+        if (len >= p->window) {
+            split_hash st;
+            split_reset(&st);
+            for (uint64_t c = 0; c < len && !hit; c++) {
+                split_update(p, &st, line[c]);
+                if (!st.full) continue;
+                uint32_t idx[11];
+                orc_split_fwd_indices(p, (uint32_t)st.h, (uint32_t)(st.h >> 32), idx);
+                hit = 1;
+                for (uint32_t j = 0; j < p->k && hit; j++) hit = test_bit(F, idx[j]);
+            }
+        }
+        hits[i] = (uint8_t)hit;
+        n_hit += (uint64_t)hit;
+    }
+    return n_hit;
+}

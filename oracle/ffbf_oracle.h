@@ -156,6 +156,15 @@ uint64_t orc_split_verify(const orc_split_params *p, const uint8_t *text, uint64
  * 2^c + 2^e bits; ffbf.cpp:487-527). */
 uint64_t orc_filter_save_deltas_bits(const uint32_t *F, uint64_t n_bits, uint32_t *out, uint64_t cap);
 
+/* Filter::operator[] (ffbf.cpp:285-308) for many lines: hits[i] = 1 iff
+ * line i (bytes[off[i] .. off[i+1])) is at least W bytes long and one of its
+ * windows, hashed from a reset over the line without restarting inside it,
+ * has all k forward bits set in F.  Returns the number of such lines. */
+uint64_t orc_query_lines(const orc_params *p, const uint8_t *bytes, const uint64_t *off, uint64_t n_lines,
+                         const uint32_t *F, uint8_t *hits);
+uint64_t orc_split_query_lines(const orc_split_params *p, const uint8_t *bytes, const uint64_t *off,
+                               uint64_t n_lines, const uint32_t *F, uint8_t *hits);
+
 #ifdef __cplusplus
 }
 #endif
