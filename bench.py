@@ -144,9 +144,10 @@ def batch_inputs(fa, name, scale=1.0):
             "n_texts": n_texts, "bytes": int(off[-1])}
 
 
-def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0):
+def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0, cpu=False):
     """Batched (one persistent launch per batch, hipGraph replay, autotuned)
-    vs the per-text launch path on the same texts; GB/s of text for each."""
+    vs the per-text launch path on the same texts; GB/s of text for each;
+    with cpu, the oracle's batched flow on the host beside them."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -211,6 +212,7 @@ def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0):
         "same_matches_as_per_text": same,
         "autotuned": best,
         "autotune_s": round(tune_s, 2),
+        "cpu_baseline": cpu_batched_baseline(d) if cpu else None,
     }
 
 
@@ -262,9 +264,14 @@ def percentile(xs, q):
 
 def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20):
     """Continuous-batching server vs the plain per-request path (one
-    ffbf_match_text per request, FIFO) on the same requests: saturation
-    throughput of each, then latency at Poisson arrivals at `load` x the
-    server's saturation rate."""
+    ffbf_match_text per request, first come first served) on the same
+    requests.  Saturation throughput of each (everything queued at once);
+    then latency under seeded Poisson arrivals, each path driven for real by a
+    submitting thread: at loads stated against the plain path's own
+    saturation rate (where both are stable), and at loads against the
+    server's (where the plain path is past its saturation, so it is reported
+    as unstable and not driven)."""
+    import queue
     import threading
 
     import numpy as np
@@ -280,20 +287,19 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         m.build()
         ms.append(m)
     reqs = d["requests"]
-    # plain per-request path: service time of each request alone
+    # plain per-request path: every request alone, back to back
     for f, t in reqs[:8]:
         ms[f].match(t)  # warm
-    service = []
     plain_res = []
+    t1 = time.perf_counter()
     for f, t in reqs:
-        t1 = time.perf_counter()
         plain_res.append(ms[f].match(t).matches)
-        service.append(time.perf_counter() - t1)
-    plain_s = float(sum(service))
+    plain_s = time.perf_counter() - t1
+    plain_rate = n_requests / plain_s
+    t1 = time.perf_counter()
     srv = fa.Server(ms, hbm_budget_bytes=budget, slots=slots, max_batch_bytes=max_batch_bytes)
-    # two warm passes over the whole request set: graph capture and the
-    # runtime's first-use costs on the copy path happen here (the first pass
-    # runs at about half the steady rate; it is reported as cold_start_gbs)
+    create_s = time.perf_counter() - t1
+    # the first pass over the new server, then one more before measuring
     t1 = time.perf_counter()
     for rid in srv.submit_many(reqs):
         srv.wait(rid)
@@ -307,15 +313,16 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     ids = srv.submit_many(reqs)
     got_all = [srv.wait(rid)[0] for rid in ids]
     sat_s = time.perf_counter() - t1
+    srv_rate = n_requests / sat_s
     same = all(np.array_equal(g["pattern_id"], p["pattern_id"]) and np.array_equal(g["offset"], p["offset"])
                for g, p in zip(got_all, plain_res))
     got_all = None
-    def poisson(ld):
-        """Poisson arrivals at ld x the saturation request rate: the server's
-        latencies, and the plain path's under the same arrivals (a FIFO
-        single server with the measured per-request service times)."""
-        lam = ld * n_requests / sat_s
-        arrivals = np.cumsum(d["gaps"]) / lam
+
+    def arrivals_at(lam):
+        return np.cumsum(d["gaps"]) / lam
+
+    def drive_server(lam):
+        arrivals = arrivals_at(lam)
         lat = [None] * n_requests
         rids = [None] * n_requests
         ready = [threading.Event() for _ in range(n_requests)]
@@ -336,24 +343,65 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
             ready[i].wait()
             _, lat[i] = srv.wait(rids[i])
         th.join()
-        elapsed = time.perf_counter() - t1
-        plain_lat, free_at = [], 0.0
-        for i in range(n_requests):
-            start = max(arrivals[i], free_at)
-            free_at = start + service[i]
-            plain_lat.append((free_at - arrivals[i]) * 1e3)
-        return lam, lat, elapsed, plain_lat
+        return lat, time.perf_counter() - t1
 
-    lam, lat, poisson_s, plain_lat = poisson(load)
+    def drive_plain(lam):
+        """The plain path at the same arrivals: a submitting thread queues
+        each request at its arrival time; one worker serves them in order,
+        one ffbf_match_text each; latency is arrival to matches in hand."""
+        arrivals = arrivals_at(lam)
+        q = queue.Queue()
+        lat = [None] * n_requests
+
+        def submitter():
+            start = time.perf_counter()
+            for i in range(n_requests):
+                dt = arrivals[i] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                q.put((i, time.perf_counter()))
+
+        th = threading.Thread(target=submitter)
+        t1 = time.perf_counter()
+        th.start()
+        for _ in range(n_requests):
+            i, t_arr = q.get()
+            f, t = reqs[i]
+            ms[f].match(t)
+            lat[i] = (time.perf_counter() - t_arr) * 1e3
+        th.join()
+        return lat, time.perf_counter() - t1
+
+    def pct(xs):
+        return round(percentile(xs, 50), 3), round(percentile(xs, 99), 3)
+
+    # both paths at the same arrival rates, below the plain path's saturation
+    same_rate = []
+    for ld in (0.5, 0.8):
+        lam = ld * plain_rate
+        s_lat, s_el = drive_server(lam)
+        p_lat, p_el = drive_plain(lam)
+        row = {"load_vs_plain_saturation": ld, "load_vs_server_saturation": round(lam / srv_rate, 4),
+               "rate_per_s": round(lam, 1), "server_p50_ms": pct(s_lat)[0], "server_p99_ms": pct(s_lat)[1],
+               "plain_p50_ms": pct(p_lat)[0], "plain_p99_ms": pct(p_lat)[1],
+               "server_gbs": round(total / s_el / 1e9, 3), "plain_gbs": round(total / p_el / 1e9, 3)}
+        same_rate.append(row)
+        log(f"serve: same arrivals at {lam:.0f} req/s ({ld} of plain saturation): server p99 {row['server_p99_ms']} ms, "
+            f"plain p99 {row['plain_p99_ms']} ms")
+    # the server at loads of its own saturation rate
     sweep = []
+    lat_at_load = None
     for ld in (0.3, 0.5, 0.7, 0.9):
-        lam_i, lat_i, s_i, plain_i = poisson(ld)
-        sweep.append({"load": ld, "rate_per_s": round(lam_i, 1), "p50_ms": round(percentile(lat_i, 50), 3),
-                      "p99_ms": round(percentile(lat_i, 99), 3), "gbs": round(total / s_i / 1e9, 3),
-                      "plain_p50_ms": round(percentile(plain_i, 50), 3),
-                      "plain_p99_ms": round(percentile(plain_i, 99), 3)})
-        log(f"serve: load {ld}: {lam_i:.0f} req/s, p50 {sweep[-1]['p50_ms']} ms, p99 {sweep[-1]['p99_ms']} ms "
-            f"(plain p99 {sweep[-1]['plain_p99_ms']} ms)")
+        lam = ld * srv_rate
+        s_lat, s_el = drive_server(lam)
+        if ld == load:
+            lat_at_load, poisson_s = s_lat, s_el
+        row = {"load": ld, "rate_per_s": round(lam, 1), "p50_ms": pct(s_lat)[0], "p99_ms": pct(s_lat)[1],
+               "gbs": round(total / s_el / 1e9, 3), "load_vs_plain_saturation": round(lam / plain_rate, 3),
+               "plain": "unstable: arrivals above its saturation rate" if lam >= plain_rate else "stable"}
+        sweep.append(row)
+        log(f"serve: load {ld}: {lam:.0f} req/s, p50 {row['p50_ms']} ms, p99 {row['p99_ms']} ms "
+            f"({row['load_vs_plain_saturation']}x the plain path's saturation rate)")
     stats = srv.stats()
     srv.close()
     for m in ms:
@@ -366,22 +414,26 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         "text_bytes": int(total),
         "filters": 8,
         "saturation_gbs": round(total / sat_s / 1e9, 3),
+        "saturation_rate_per_s": round(srv_rate, 1),
         "cold_start_gbs": round(total / cold_s / 1e9, 3),
+        "server_create_s": round(create_s, 3),
         "plain_gbs": round(total / plain_s / 1e9, 3),
+        "plain_saturation_rate_per_s": round(plain_rate, 1),
         "speedup_vs_plain": round(plain_s / sat_s, 2),
         "same_matches_as_plain": bool(same),
-        "poisson_rate_per_s": round(lam, 1),
         "poisson_load": load,
-        "p50_ms": round(percentile(lat, 50), 3),
-        "p99_ms": round(percentile(lat, 99), 3),
+        "poisson_rate_per_s": round(load * srv_rate, 1),
+        "p50_ms": round(percentile(lat_at_load, 50), 3),
+        "p99_ms": round(percentile(lat_at_load, 99), 3),
         "poisson_gbs": round(total / poisson_s / 1e9, 3),
-        "plain_p50_ms": round(percentile(plain_lat, 50), 3),
-        "plain_p99_ms": round(percentile(plain_lat, 99), 3),
+        "same_arrivals": same_rate,
         "load_sweep": sweep,
         "batches": int(stats["batches"]),
         "hbm_budget_bytes": budget,
         "slots": slots,
         "max_batch_bytes": max_batch_bytes,
+        "latency_drivers": "both paths driven by a submitting thread at seeded Poisson arrivals; latency from "
+                           "submission to matches in hand",
     }
 
 
@@ -502,7 +554,28 @@ def copy_kernel_rate(fa, nbytes=4 << 30, reps=5):
     return round(2 * nbytes / float(np.median(ts)) / 1e9, 1)
 
 
-def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
+def cpu_threads():
+    """Host threads of the threaded CPU baseline: the CPUs this process may
+    run on, at most 16 -- one GPU's share of the host's cores on the MI355X
+    boxes, whatever the machine's total count is."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline(wl, sbox, pb, po, text, budget_s=12.0):
     """Oracle (C restatement of the reference path) on a bounded text sample."""
     from oracle import Oracle
 
@@ -511,10 +584,15 @@ def cpu_baseline(wl, sbox, pb, po, text, budget_s=20.0):
 
 
 def _cpu_baseline(o, n_pat, pb, po, text, budget_s):
+    """The oracle's whole flow over one text on 1 thread and on cpu_threads()
+    threads, each stage timed on its own: build (all patterns), scan with
+    feed-forward (the whole text when the budget allows, else its first
+    bytes, the time scaled to the whole), pattern pass, verification (scaled
+    like the scan).  The scan is the median of three after one warm run."""
     import numpy as np
 
     n_text = text.size
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     t0 = time.perf_counter()
     F = o.build(pb, po)
     t_build = time.perf_counter() - t0
@@ -522,52 +600,113 @@ def _cpu_baseline(o, n_pat, pb, po, text, budget_s):
     probe = text[: 8 << 20]
     t0 = time.perf_counter()
     o.scan(probe, F)
-    rate1 = probe.size / (time.perf_counter() - t0)
-    # each sample is scanned once untimed, then 5 times timed (median)
-    n1 = int(min(text.size, max(8 << 20, rate1 * budget_s * 0.25 / 6))) & ~4095
-    nT = int(min(text.size, max(8 << 20, rate1 * threads * budget_s * 0.5 / 6))) & ~4095
+    rate1 = probe.size / max(time.perf_counter() - t0, 1e-9)
+    n1 = int(min(text.size, max(8 << 20, rate1 * budget_s * 0.3 / 4)))
+    nT = int(min(text.size, max(8 << 20, rate1 * threads * budget_s * 0.5 / 4)))
+    n1 = n1 if n1 == text.size else n1 & ~4095
+    nT = nT if nT == text.size else nT & ~4095
     res = {}
     for label, n, th in (("1", n1, 1), ("T", nT, threads)):
         sample = np.ascontiguousarray(text[:n])
         o.scan(sample, F, threads=th)  # warm
         runs = []
-        for _ in range(5):
+        for _ in range(3):
             t0 = time.perf_counter()
             R, cand = o.scan(sample, F, threads=th)
             runs.append(time.perf_counter() - t0)
-        t_scan = float(np.median(runs))
+        t_scan = float(np.median(runs)) * (n_text / n)
         t0 = time.perf_counter()
         surv = o.pattern_pass(pb, po, R)
         t_pass = time.perf_counter() - t0
         t0 = time.perf_counter()
         o.verify(sample, pb, po, cand, surv)
-        t_ver = time.perf_counter() - t0
-        # whole job over the full text: build once + per-byte scan rate + pass + verify
-        total = t_build + t_scan * (n_text / n) + t_pass + t_ver
-        res[label] = {"n": n, "threads": th, "scan_s": t_scan, "pass_s": t_pass, "verify_s": t_ver,
-                      "gbs": n_text / total / 1e9, "scan_gbs": n / t_scan / 1e9}
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+        t_ver = (time.perf_counter() - t0) * (n_text / n)
+        total = t_build + t_scan + t_pass + t_ver
+        res[label] = {"threads": th, "sample_bytes": n, "build_s": round(t_build, 4), "scan_s": round(t_scan, 4),
+                      "pattern_pass_s": round(t_pass, 4), "verify_s": round(t_ver, 4), "total_s": round(total, 4),
+                      "gbs": round(n_text / total / 1e9, 4), "scan_gbs": round(n_text / t_scan / 1e9, 4)}
     T = res["T"]
     return {
-        "value": round(T["gbs"], 4),
+        "value": T["gbs"],
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle build of all {n_pat} patterns ({t_build:.2f} s) + scan of the first "
-                   f"{T['n'] >> 20} MiB of the text on {threads} threads ({T['scan_gbs']:.3f} GB/s) "
-                   f"(median of 5 after one warm run) + pattern pass + verify, scan time scaled to the full "
-                   f"{n_text >> 20} MiB; "
-                   f"1 thread: {res['1']['gbs']:.4f} GB/s (scan {res['1']['scan_gbs']:.3f} GB/s, median of 5, on "
-                   f"{res['1']['n'] >> 20} MiB); host {cpu_model}, {os.cpu_count()} logical CPUs"),
-        "single_thread_value": round(res["1"]["gbs"], 4),
+        "sample": (f"oracle build of all {n_pat} patterns ({t_build:.2f} s) + scan with feed-forward of "
+                   + ("the whole text" if T["sample_bytes"] == n_text else f"the first {T['sample_bytes'] >> 20} MiB")
+                   + f" on {threads} threads ({T['scan_gbs']:.3f} GB/s, median of 3 after one warm run, scaled to "
+                   f"{n_text >> 20} MiB) + pattern pass + verification; 1 thread: {res['1']['gbs']:.4f} GB/s on "
+                   + ("the whole text" if res["1"]["sample_bytes"] == n_text else
+                      f"the first {res['1']['sample_bytes'] >> 20} MiB")
+                   + f"; host {cpu_model()}, {os.cpu_count()} logical CPUs, {threads} used"),
+        "single_thread_value": res["1"]["gbs"],
+        "stages": res,
     }
+
+
+def cpu_batched_baseline(d, budget_s=8.0):
+    """The oracle over a batch the way the batched path runs it: every filter
+    built once; every text scanned with feed-forward into its filter's R; one
+    pattern pass per filter; each text's candidates verified against its
+    filter's survivors.  On 1 thread and on cpu_threads() threads (texts
+    spread over a pool; the oracle's C calls run outside the interpreter
+    lock).  Texts are taken in order until the scan budget is spent; scan and
+    verification times are scaled to all the batch's bytes."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    from oracle import Oracle
+
+    threads = cpu_threads()
+    off, route, buf = d["offsets"], d["route"], d["texts"]
+    n_texts, total = d["n_texts"], d["bytes"]
+    orcs = [Oracle(d["W"], d["k"], log2, d["sbox"]) for _, _, log2 in d["sets"]]
+    t0 = time.perf_counter()
+    Fs = [o.build(pb, po) for o, (pb, po, _) in zip(orcs, d["sets"])]
+    t_build = time.perf_counter() - t0
+    active = sorted(set(int(f) for f in route))
+
+    def text(i):
+        return buf[int(off[i]):int(off[i + 1])]
+
+    def scan_one(i):
+        f = int(route[i])
+        R, cand = orcs[f].scan(text(i), Fs[f])
+        return i, f, R, cand
+
+    res = {}
+    for label, th in (("1", 1), ("T", threads)):
+        R_union = {f: np.zeros_like(Fs[f]) for f in active}
+        cands, sample = [], 0
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(th) as ex:
+            i = 0
+            while i < n_texts and (time.perf_counter() - t0 < budget_s * 0.4 or i < 8):
+                for j, f, R, cand in ex.map(scan_one, range(i, min(n_texts, i + 4 * th))):
+                    R_union[f] |= R
+                    cands.append((j, f, cand))
+                    sample += int(off[j + 1] - off[j])
+                i = min(n_texts, i + 4 * th)
+            t_scan = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            surv = dict(zip(active, ex.map(lambda f: orcs[f].pattern_pass(d["sets"][f][0], d["sets"][f][1],
+                                                                           R_union[f]), active)))
+            t_pass = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            list(ex.map(lambda c: orcs[c[1]].verify(text(c[0]), d["sets"][c[1]][0], d["sets"][c[1]][1], c[2],
+                                                    surv[c[1]]), cands))
+            t_ver = time.perf_counter() - t0
+        scale = total / max(sample, 1)
+        tot = t_build + t_scan * scale + t_pass + t_ver * scale
+        res[label] = {"threads": th, "sample_bytes": sample, "sample_texts": len(cands), "build_s": round(t_build, 4),
+                      "scan_s": round(t_scan * scale, 4), "pattern_pass_s": round(t_pass, 4),
+                      "verify_s": round(t_ver * scale, 4), "total_s": round(tot, 4), "gbs": round(total / tot / 1e9, 4)}
+    return {"value": res["T"]["gbs"], "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle: {len(d['sets'])} filters built, then the first {res['T']['sample_texts']} of "
+                       f"{n_texts} texts ({res['T']['sample_bytes'] >> 20} MiB) scanned on {threads} threads, one "
+                       f"pattern pass per filter, per-text verification; scan and verification scaled to "
+                       f"{total >> 20} MiB; 1 thread: {res['1']['gbs']:.4f} GB/s on the first "
+                       f"{res['1']['sample_texts']} texts"),
+            "single_thread_value": res["1"]["gbs"], "stages": res}
 
 
 def split_inputs(fa, wl):
@@ -648,6 +787,90 @@ def traffic_from_profiles(name):
         return d.get(name)
     except (OSError, ValueError):
         return None
+
+
+def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
+    """The single-text path over a device-resident text: each step builds F,
+    scans with feed-forward, runs the pattern pass, verifies and brings the
+    sorted matches to the host.  Returns the bench fields and the inputs."""
+    import numpy as np
+
+    wl = WORKLOADS[name]
+    alpha, W, k, log2, n_pat, lmin, lmax, n_text, plants, seed = wl
+    t0 = time.perf_counter()
+    sbox, pb, po, text = make_inputs(fa, wl)
+    log(f"{name}: inputs generated in {time.perf_counter() - t0:.1f} s")
+    dev = fa.DeviceBuffer.from_host(text)
+    m = fa.Matcher(W, k, log2, sbox)
+    m.set_patterns(pb, po)
+    m.set_scan_mode(mode)
+
+    def step():
+        m.build()
+        m.run_device(dev.ptr, n_text)
+        return m.matches()
+
+    for _ in range(warmup):
+        step()
+    m.synchronize()
+    scan_ms, stages = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = step()
+        t = m.timings()
+        scan_ms.append(t["scan_kernel_ms"])
+        stages.append((t["build_ms"], t["scan_ms"], t["pattern_pass_ms"], t["verify_ms"]))
+    m.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"{name}: timed {steps} steps in {elapsed:.3f} s")
+    ms_per_step = elapsed * 1e3 / steps
+    value = n_text / (ms_per_step * 1e-3) / 1e9
+    t_scan = float(np.mean(scan_ms))
+    achieved = n_text / (t_scan * 1e-3) / 1e9
+    st = np.mean(np.array(stages), axis=0)
+    tm = m.timings()
+    ceiling = probe_ceiling(fa, m, k, n_text, W, t_scan)
+    out = {
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded counter-based generator)",
+        "config": describe(name, wl),
+        "hbm_frac_of_step": round(value / HBM_PEAK_GBS, 5),
+        "roofline": {
+            "kernel": "k_scan",
+            "bound": "hbm",
+            # what holds the kernel below that roof, from its L2 and SQ counters
+            "limiter": ("L2 random-request rate" if tm["scan_mode"] in (0, 3) else "VALU/LDS instruction issue"),
+            "achieved": round(achieved, 3),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic_from_profiles(name),
+            "avg_launch_ms": round(t_scan, 4),
+            "algorithmic_bytes": n_text,
+            "copy_kernel_gbs": copy_kernel_rate(fa) if copy_rate else None,
+        },
+        "stages_ms": {"build": round(st[0], 4), "scan": round(st[1], 4), "pattern_pass": round(st[2], 4),
+                      "verify": round(st[3], 4)},
+        "scan_mode": {0: "F probed in L2/HBM", 1: "LDS fold prefilter", 2: "filter in LDS",
+                      3: "L2-resident global fold"}.get(tm["scan_mode"]),
+        "fold_log2_bits": tm["fold_log2_bits"],
+        "probes": tm["probes"],
+        "l2_probe_ceiling": ceiling,
+        "matches": int(res.size),
+        "candidates": int(m.n_candidates()),
+    }
+    m.close()
+    dev.free()
+    return out, (sbox, pb, po, text)
 
 
 def main() -> int:
@@ -741,88 +964,19 @@ def main() -> int:
                           "batched": r}), flush=True)
         return 0
 
-    wl = WORKLOADS[args.workload]
-    alpha, W, k, log2, n_pat, lmin, lmax, n_text, plants, seed = wl
-    t0 = time.perf_counter()
-    sbox, pb, po, text = make_inputs(fa, wl)
-    log(f"inputs generated in {time.perf_counter() - t0:.1f} s")
-    dev = fa.DeviceBuffer.from_host(text)
-    m = fa.Matcher(W, k, log2, sbox)
-    m.set_patterns(pb, po)
-    m.set_scan_mode(args.mode)
-
-    def step():
-        m.build()
-        m.run_device(dev.ptr, n_text)
-        return m.matches()
-
-    for _ in range(args.warmup):
-        step()
-    log("warmup done")
-    m.synchronize()
-    scan_ms, stages = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = step()
-        t = m.timings()
-        scan_ms.append(t["scan_kernel_ms"])
-        stages.append((t["build_ms"], t["scan_ms"], t["pattern_pass_ms"], t["verify_ms"]))
-    m.synchronize()
-    elapsed = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {elapsed:.3f} s")
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = n_text / (ms_per_step * 1e-3) / 1e9
-    t_scan = float(np.mean(scan_ms))
-    achieved = n_text / (t_scan * 1e-3) / 1e9
-    st = np.mean(np.array(stages), axis=0)
-    mode = m.timings()
-    ceiling = probe_ceiling(fa, m, k, n_text, W, t_scan)
-    copy_gbs = copy_kernel_rate(fa)
-    out = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic (seeded counter-based generator)",
-        "config": describe(args.workload, wl),
-        "hbm_frac_of_step": round(value / HBM_PEAK_GBS, 5),
-        "roofline": {
-            "kernel": "k_scan",
-            "bound": "hbm",
-            # what holds the kernel below that roof, from its L2 and SQ counters
-            "limiter": ("L2 random-request rate" if mode["scan_mode"] in (0, 3) else "VALU/LDS instruction issue"),
-            "achieved": round(achieved, 3),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles(args.workload),
-            "avg_launch_ms": round(t_scan, 4),
-            "algorithmic_bytes": n_text,
-            "copy_kernel_gbs": copy_gbs,
-        },
-        "stages_ms": {"build": round(st[0], 4), "scan": round(st[1], 4), "pattern_pass": round(st[2], 4),
-                      "verify": round(st[3], 4)},
-        "scan_mode": {0: "F probed in L2/HBM", 1: "LDS fold prefilter", 2: "filter in LDS",
-                      3: "L2-resident global fold"}.get(mode["scan_mode"]),
-        "fold_log2_bits": mode["fold_log2_bits"],
-        "l2_probe_ceiling": ceiling,
-        "matches": int(res.size),
-        "candidates": int(m.n_candidates()),
-    }
-    m.close()
-    dev.free()
+    out, inputs = run_plain(fa, args.workload, args.steps, args.warmup, args.mode)
+    out = {"metric": METRIC, **out}
     if args.workload == "plain" and not args.no_batched:
+        log("config 1 leg (ascii10k-64m) ...")
+        c1, c1_in = run_plain(fa, "ascii10k-64m", 20, 3, -1, copy_rate=False)
+        if not args.no_cpu:
+            c1["cpu_baseline"] = cpu_baseline(WORKLOADS["ascii10k-64m"], *c1_in, budget_s=6.0)
+        out["config1"] = c1
         log("batched leg (batch-4096texts-1set) ...")
-        out["batched"] = run_batched(fa, "batch-4096texts-1set", steps=10)
+        out["batched"] = run_batched(fa, "batch-4096texts-1set", steps=10, cpu=not args.no_cpu)
         log("multi-filter batched leg (batch-2048texts-64sets) ...")
-        out["batched_multi_filter"] = run_batched(fa, "batch-2048texts-64sets", steps=5, per_text_reps=1)
+        out["batched_multi_filter"] = run_batched(fa, "batch-2048texts-64sets", steps=5, per_text_reps=1,
+                                                  cpu=not args.no_cpu)
         log("serving leg (serve-8sets) ...")
         out["serving"] = run_serving(fa, n_requests=600)
         log("ingest leg (host memory / file) ...")
@@ -831,7 +985,15 @@ def main() -> int:
         out["split_method"] = run_split(fa, "split-1m-2g-lines", steps=10, cpu=not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(wl, sbox, pb, po, text)
+        out["cpu_baseline"] = cpu_baseline(WORKLOADS[args.workload], *inputs)
+        if args.workload == "plain" and not args.no_batched:
+            # the CPU stages of configs 1-4 side by side (1 thread / cpu_threads())
+            out["cpu_table"] = {
+                "config1 ascii10k-64m": out["config1"]["cpu_baseline"]["stages"],
+                "config2 plain": out["cpu_baseline"]["stages"],
+                "config3 batch-4096texts-1set": out["batched"]["cpu_baseline"]["stages"],
+                "config4 batch-2048texts-64sets": out["batched_multi_filter"]["cpu_baseline"]["stages"],
+            }
     print(json.dumps(out), flush=True)
     return 0
 

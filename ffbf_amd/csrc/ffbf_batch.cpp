@@ -16,6 +16,8 @@
 #include <cstring>
 #include <vector>
 
+#include <cmath>
+
 #include "ffbf_ctx.h"
 
 using namespace ffbf;
@@ -374,7 +376,15 @@ int batch_reserve(ffbf_batch *b, uint32_t max_texts, uint64_t max_bytes)
     // chunks: every text adds at most two partial wave-iterations and one
     // partial chunk beyond its share of the bytes
     const uint64_t chunks = max_bytes / (16ull * kIterUnits * ci) + 3ull * max_texts + 16u;
-    const uint64_t cand = std::max<uint64_t>(1 << 16, max_bytes >> 10);
+    // candidates: twice the false-positive windows the densest filter is
+    // expected to pass (fill^k per window), at least 1 per KiB and at most 1
+    // per 16 bytes; a batch that overflows its list regrows it, which stalls
+    // the device and re-captures the graph (filters of 16 bits per pattern
+    // with k=3 pass ~0.5% of windows)
+    double fp = 0.0;
+    for (ffbf_ctx *c : b->filters) fp = std::max(fp, std::pow(f_fill(c), (double)c->params.k));
+    const uint64_t fp_want = std::min<uint64_t>((uint64_t)(2.0 * fp * (double)max_bytes), max_bytes >> 4);
+    const uint64_t cand = std::max<uint64_t>(std::max<uint64_t>(1 << 16, max_bytes >> 10), fp_want);
     bool moved = false;
     if (b->d_texts.cap < max_texts) {
         RET_IF(b->d_texts.ensure(std::max<uint64_t>(max_texts, 1024), "batch text descriptors"));

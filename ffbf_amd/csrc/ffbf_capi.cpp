@@ -464,7 +464,12 @@ int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n)
     if (!dev_text && n) return fail(FFBF_EINVAL, "null text");
     c->text = dev_text;
     c->text_n = n;
-    const uint64_t want = std::max<uint64_t>(65536, n >> 10);
+    // room for twice the expected false-positive windows (fill^k each), at
+    // least 1 per KiB and at most 1 per 16 bytes up front: an overflowing
+    // list makes the path scan again
+    const double fp = std::pow(f_fill(c), (double)c->params.k);
+    const uint64_t fp_want = std::min<uint64_t>((uint64_t)(2.0 * fp * (double)n), n >> 4);
+    const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(65536, n >> 10), fp_want);
     RET_IF(ensure_cand(c, want));
     if (!c->matches.p) RET_IF(c->matches.ensure(65536, "matches"));
     c->scanned = c->passed = c->verified = false;

@@ -235,4 +235,6 @@ inline bool fold_passes_many(double fill, uint32_t P)
 int use_device(ffbf_ctx *c);
 // Workgroup size of the context's scan launches (its scan mode and tuning).
 uint32_t scan_block(ffbf_ctx *c);
+// Expected fraction of F's bits that are set (exact after a filter load).
+double f_fill(ffbf_ctx *c);
 }  // namespace ffbf

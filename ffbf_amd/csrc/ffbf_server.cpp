@@ -178,6 +178,20 @@ int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_
             sl.d_text = nullptr;
             return fail(FFBF_ENOMEM, "server slot buffers");
         }
+        // one batch now, a zero text per filter: the slot's graph is captured
+        // and instantiated here (its buffers never move, so requests replay
+        // it) instead of in the first request's latency
+        const uint64_t wn = std::min<uint64_t>(4096, sl.cap / n_filters) & ~15ull;
+        if (wn >= 64) {
+            HIP_TRY(launch_fill32((uint32_t *)sl.d_text, wn * n_filters / 4u, 0u, batch_stream(sl.batch)));
+            std::vector<uint64_t> woff(n_filters + 1);
+            std::vector<uint32_t> wroute(n_filters);
+            for (uint32_t f = 0; f <= n_filters; f++) woff[f] = (uint64_t)f * wn;
+            for (uint32_t f = 0; f < n_filters; f++) wroute[f] = f;
+            uint64_t nm = 0;
+            RET_IF(ffbf_batch_set_texts(sl.batch, sl.d_text, woff.data(), wroute.data(), n_filters));
+            RET_IF(ffbf_batch_run(sl.batch, &nm));
+        }
     }
     SchedConfig sc;
     sc.hbm_budget_bytes = s->cfg.hbm_budget_bytes;
