@@ -503,29 +503,42 @@ def run_ingest(fa, reps=3):
 
 
 def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
-    """For the scans whose prefilter lives in global memory (an L2-resident
-    fold of F, or F itself): random 4-byte probes per second the scan made,
-    beside the chip's random-gather rate into an array of the prefilter's size
-    measured now.  Probes per window follow from the prefilter's fill f: bit 0
-    for every window, bit 1 where bit 0 is set, bits 2..k-1 together where
-    both are (F's own bits after a full fold hit add ~f^k)."""
+    """For the scans that probe a prefilter in global memory (an L2-resident
+    fold of F, behind an LDS fold or not, or F itself): random 4-byte probes
+    per second the scan made, beside the chip's random-gather rate into an
+    array of the prefilter's size measured now.  Probes per window follow
+    from the prefilter's fill f: bit 0 for every window, bit 1 where bit 0 is
+    set, bits 2..k-1 together where both are (F's own bits after a full fold
+    hit add ~f^k); behind an LDS fold of fill l, only the share l^P of
+    windows passing its P up-front bits probe at all."""
     import numpy as np
 
     t = m.timings()
-    if t["scan_mode"] not in (0, 3):
+    if t["scan_mode"] not in (0, 3, 4):
         return None
     F = m.filter(fa.FILTER_FORWARD)
-    words = 1 << (t["fold_log2_bits"] - 5)
-    pre = np.bitwise_or.reduce(F.reshape(-1, words), axis=0) if words < F.size else F
-    f = float(np.bitwise_count(pre).sum()) / (pre.size * 32.0)
-    per_window = 1.0 + (f if k > 1 else 0.0) + (max(0, k - 2) * f * f)
+
+    def fold(log2_bits):
+        words = 1 << (log2_bits - 5)
+        return np.bitwise_or.reduce(F.reshape(-1, words), axis=0) if words < F.size else F
+
+    def fill(a):
+        return float(np.bitwise_count(a).sum()) / (a.size * 32.0)
+
+    pre_log2 = t["l2_fold_log2_bits"] if t["scan_mode"] in (3, 4) else (F.size * 32).bit_length() - 1
+    pre = fold(pre_log2) if t["scan_mode"] in (3, 4) else F
+    f = fill(pre)
+    lds_pass = fill(fold(t["fold_log2_bits"])) ** t["probes"] if t["scan_mode"] == 4 else 1.0
+    per_window = lds_pass * (1.0 + (f if k > 1 else 0.0) + (max(0, k - 2) * f * f))
     windows = max(0, n_text - W + 1)
     rate = windows * per_window / (t_scan_ms * 1e-3)
-    _, ceiling = fa.gather_bench(t["fold_log2_bits"] - 3, steps=16)
+    _, ceiling = fa.gather_bench(pre_log2 - 3, steps=16)
     return {
-        "prefilter": "L2-resident fold of F" if t["scan_mode"] == 3 else "F",
+        "prefilter": {0: "F", 3: "L2-resident fold of F",
+                      4: "L2-resident fold of F behind an LDS fold"}[t["scan_mode"]],
         "prefilter_bytes": pre.size * 4,
         "prefilter_fill": round(f, 5),
+        "lds_pass": round(lds_pass, 4),
         "probes_per_window": round(per_window, 4),
         "scan_probes_per_s": round(rate, 1),
         "gather_probes_per_s": round(ceiling, 1),
@@ -848,7 +861,7 @@ def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
             "kernel": "k_scan",
             "bound": "hbm",
             # what holds the kernel below that roof, from its L2 and SQ counters
-            "limiter": ("L2 random-request rate" if tm["scan_mode"] in (0, 3) else "VALU/LDS instruction issue"),
+            "limiter": ("L2 random-request rate" if tm["scan_mode"] in (0, 3, 4) else "VALU/LDS instruction issue"),
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -861,8 +874,9 @@ def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
         "stages_ms": {"build": round(st[0], 4), "scan": round(st[1], 4), "pattern_pass": round(st[2], 4),
                       "verify": round(st[3], 4)},
         "scan_mode": {0: "F probed in L2/HBM", 1: "LDS fold prefilter", 2: "filter in LDS",
-                      3: "L2-resident global fold"}.get(tm["scan_mode"]),
+                      3: "L2-resident global fold", 4: "LDS fold, then L2-resident global fold"}.get(tm["scan_mode"]),
         "fold_log2_bits": tm["fold_log2_bits"],
+        "l2_fold_log2_bits": tm["l2_fold_log2_bits"],
         "probes": tm["probes"],
         "l2_probe_ceiling": ceiling,
         "matches": int(res.size),

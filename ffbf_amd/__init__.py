@@ -41,7 +41,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY, ECANCELED = -1, -2, -3, -4, -5, -6
 FILTER_FORWARD, FILTER_REVERSE = 0, 1
-MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS, MODE_GFOLD = -1, 0, 1, 2, 3
+MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS, MODE_GFOLD, MODE_HYBRID = -1, 0, 1, 2, 3, 4
 ALPHA_ASCII, ALPHA_DNA_GC40 = 0, 1
 
 
@@ -88,6 +88,7 @@ class Timings(ctypes.Structure):
         ("probes", ctypes.c_uint32),
         ("block", ctypes.c_uint32),
         ("ingest_ms", ctypes.c_float),
+        ("l2_fold_log2_bits", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -118,7 +118,7 @@ int refresh_filters(ffbf_batch *b)
     b->has_lds = b->has_global = false;
     for (uint32_t f = 0; f < nf; f++) {
         const ffbf_ctx *c = b->filters[f];
-        const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
+        const bool lds = mode_in_lds(c->mode);
         // an R gathered in LDS sits right after the filter
         ScanArgs probe{};
         filter_scan_args(c, probe);
@@ -132,7 +132,7 @@ int refresh_filters(ffbf_batch *b)
         p = 3;
         for (uint32_t f = 0; f < nf; f++) {
             const ffbf_ctx *c = b->filters[f];
-            if (c->mode == MODE_FOLD || c->mode == MODE_LDS) p = std::min(p, std::max(1u, c->probes));
+            if (mode_in_lds(c->mode)) p = std::min(p, std::max(1u, c->probes));
         }
     }
     b->probes_eff = std::max(1u, std::min({p, 3u, b->k}));
@@ -146,7 +146,7 @@ int refresh_filters(ffbf_batch *b)
         const ffbf_ctx *c = b->filters[f];
         ScanArgs &a = sa[f];
         a = ScanArgs{};
-        const bool lds = c->mode == MODE_FOLD || c->mode == MODE_LDS;
+        const bool lds = mode_in_lds(c->mode);
         filter_scan_args(c, a);
         // region slots: LDS-class launch [0, half), global-class launch [half, all)
         const uint32_t half = b->cstage.n_regions / 2u;
@@ -156,7 +156,9 @@ int refresh_filters(ffbf_batch *b)
             b->cstage.fill(a, half, b->cstage.n_regions);
         a.counters = b->counters.p;
         a.P = lds ? b->probes_eff : 1u;
-        a.f_first = (c->mode == MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)) ? 1u : 0u;
+        a.f_first = (c->mode == MODE_HYBRID || (c->mode == MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)))
+                        ? 1u
+                        : 0u;
         VerifyArgs &v = va[f];
         v = VerifyArgs{};
         v.bytes = c->bytes.p;
@@ -517,7 +519,7 @@ int ffbf_batch_set_texts(ffbf_batch *b, const uint8_t *dev_texts, const uint64_t
     // chunk class of a filter: 0 when its prefilter is held in LDS
     auto cls = [&](uint32_t f) -> uint32_t {
         const ffbf_ctx *c = b->filters[f];
-        return (c->mode == MODE_FOLD || c->mode == MODE_LDS) ? 0u : 1u;
+        return mode_in_lds(c->mode) ? 0u : 1u;
     };
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         const uint32_t cx = cls(tx[x].filter), cy = cls(tx[y].filter);

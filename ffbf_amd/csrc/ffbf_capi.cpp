@@ -110,7 +110,7 @@ int choose_mode_split(ffbf_ctx *c)
 {
     const ffbf::SplitMap &m = c->smap;
     const uint32_t S = m.s;
-    if (c->mode_req == ffbf::MODE_LDS || c->mode_req == ffbf::MODE_GFOLD)
+    if (c->mode_req == ffbf::MODE_LDS || c->mode_req == ffbf::MODE_GFOLD || c->mode_req == ffbf::MODE_HYBRID)
         return fail(FFBF_EINVAL, "split filter: scan mode must be -1, 0 or 1");
     if (c->mode_req == ffbf::MODE_FOLD && S == 0)
         return fail(FFBF_EINVAL, "split filter with s = 0 has no cache part to fold");
@@ -180,6 +180,7 @@ uint32_t auto_probes(const ffbf_ctx *c, double fill)
 int choose_mode(ffbf_ctx *c)
 {
     c->generation++;
+    c->gfold_words = 0;
     if (c->split) return choose_mode_split(c);
     const uint32_t max_words = ffbf::max_fold_words(c->device);
     const uint32_t k = c->params.k;
@@ -201,6 +202,7 @@ int choose_mode(ffbf_ctx *c)
         HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, words, c->stream));
         mode = ffbf::MODE_GFOLD;
         c->pre_words = words;
+        c->gfold_words = words;
     } else {
         uint32_t words = max_words;
         if (c->fold_log2_req) {
@@ -213,7 +215,7 @@ int choose_mode(ffbf_ctx *c)
         const double ff = fold_fill(c, (uint64_t)words * 32u);
         uint32_t choice = ffbf::MODE_FOLD;
         uint32_t gl2 = 0;
-        if (c->mode_req != ffbf::MODE_FOLD) {
+        if (c->mode_req != ffbf::MODE_FOLD && c->mode_req != ffbf::MODE_HYBRID) {
             // Expected scan cost per window (picoseconds of the whole chip),
             // fitted to MI355X measurements: the LDS stage (hash + up to 3
             // fold bits) ~1.25; one random probe ~3.8 while its array fits
@@ -233,23 +235,40 @@ int choose_mode(ffbf_ctx *c)
             const double queued = early ? pass * f : pass;
             const double cost_fold = kLds + probed * far + queued * kQueue;
             const double cost_global = (1.0 + f + f * f) * far;
-            double cost_gfold = 1e30;
+            double cost_gfold = 1e30, cost_hybrid = 1e30;
             if (!f_in_l2) {
                 gl2 = l2_fold_log2(c);
                 const double g = fold_fill(c, 1ull << gl2);
                 cost_gfold = (1.0 + g + g * g) * kL2 + std::pow(g, (double)k) * kFar;
+                // the LDS stage overlaps the L2 probes of the windows it
+                // lets through, whichever of the two takes longer
+                cost_hybrid = std::max(kLds, pass * cost_gfold);
             }
-            if (cost_global < cost_fold && cost_global <= cost_gfold)
+            if (cost_global < cost_fold && cost_global <= cost_gfold && cost_global <= cost_hybrid)
                 choice = ffbf::MODE_GLOBAL;
+            else if (cost_hybrid < cost_fold && cost_hybrid < cost_gfold)
+                choice = ffbf::MODE_HYBRID;
             else if (cost_gfold < cost_fold)
                 choice = ffbf::MODE_GFOLD;
+        } else if (c->mode_req == ffbf::MODE_HYBRID) {
+            if (c->f_words * 4ull <= (4ull << 20)) return fail(FFBF_EINVAL, "scan mode 4: filter already fits L2");
+            gl2 = l2_fold_log2(c);
+            if ((1ull << (gl2 - 5)) <= words) return fail(FFBF_EINVAL, "scan mode 4: L2 fold not larger than the LDS fold");
+            choice = ffbf::MODE_HYBRID;
         }
-        if (choice == ffbf::MODE_FOLD) {
+        if (choice == ffbf::MODE_FOLD || choice == ffbf::MODE_HYBRID) {
             RET_IF(c->fold.ensure(words, "fold"));
             HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
             mode = ffbf::MODE_FOLD;
             c->pre_words = words;
             c->pre_fill = ff;
+            if (choice == ffbf::MODE_HYBRID) {
+                const uint32_t gw = 1u << (gl2 - 5);
+                RET_IF(c->gfold.ensure(gw, "global fold"));
+                HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, gw, c->stream));
+                c->gfold_words = gw;
+                mode = ffbf::MODE_HYBRID;
+            }
         } else if (choice == ffbf::MODE_GLOBAL) {
             mode = ffbf::MODE_GLOBAL;
             c->pre_words = (uint32_t)c->f_words;
@@ -259,6 +278,7 @@ int choose_mode(ffbf_ctx *c)
             HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, gw, c->stream));
             mode = ffbf::MODE_GFOLD;
             c->pre_words = gw;
+            c->gfold_words = gw;
         }
     }
     RET_IF(st);
@@ -267,14 +287,14 @@ int choose_mode(ffbf_ctx *c)
     if (c->probes_req)
         c->probes = std::min(c->probes_req, k);
     else
-        c->probes = (mode == ffbf::MODE_FOLD || mode == ffbf::MODE_LDS) ? auto_probes(c, c->pre_fill) : 1u;
+        c->probes = ffbf::mode_in_lds(mode) ? auto_probes(c, c->pre_fill) : 1u;
     return FFBF_OK;
 }
 
 uint32_t scan_block(ffbf_ctx *c)
 {
     if (c->split) return c->mode == ffbf::MODE_FOLD ? 1024u : 256u;
-    if (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) return c->block_req == 512u ? 512u : 1024u;
+    if (ffbf::mode_in_lds(c->mode)) return c->block_req == 512u ? 512u : 1024u;
     return 256u;
 }
 
@@ -336,17 +356,25 @@ void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
     a.k = c->params.k;
     a.mask = c->mask;
     switch (c->mode) {
-    case ffbf::MODE_FOLD: a.pre = c->fold.p; break;
+    case ffbf::MODE_FOLD:
+    case ffbf::MODE_HYBRID: a.pre = c->fold.p; break;
     case ffbf::MODE_GFOLD: a.pre = c->gfold.p; break;
     default: a.pre = c->F.p; break;
     }
     a.pre_words = c->pre_words;
     a.pre_mask = c->mode == ffbf::MODE_GLOBAL ? c->mask : (uint32_t)((uint64_t)c->pre_words * 32u - 1u);
     a.pre_log2w = pow2_at_least(c->pre_words);
-    a.P = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
+    a.P = ffbf::mode_in_lds(c->mode) ? std::min(c->probes, 3u) : 1u;
     a.pre_exact = (c->mode == ffbf::MODE_GLOBAL || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
-    a.lds = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? 1u : 0u;
+    a.lds = ffbf::mode_in_lds(c->mode) ? 1u : 0u;
     a.f_first = (c->mode == ffbf::MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)) ? 1u : 0u;
+    if (c->mode == ffbf::MODE_HYBRID) {
+        // the L2-resident fold's first bit before queueing, its other bits
+        // in the drain, then F
+        a.pre2 = c->gfold.p;
+        a.pre2_mask = (uint32_t)((uint64_t)c->gfold_words * 32u - 1u);
+        a.f_first = 1u;
+    }
     // whole F in LDS and small: R is gathered beside it in LDS.  Below 2^17
     // bits the candidates' atomics crowd onto a few cache lines of a global
     // R (a 2^15-bit filter with dense candidates scanned 2.9x slower for
@@ -791,7 +819,7 @@ int ffbf_build(ffbf_ctx *c)
 int ffbf_set_scan_mode(ffbf_ctx *c, int mode, uint32_t fold_log2_bits)
 {
     RET_IF(check_ctx(c));
-    if (mode < -1 || mode > 3) return fail(FFBF_EINVAL, "mode must be -1..3");
+    if (mode < -1 || mode > 4) return fail(FFBF_EINVAL, "mode must be -1..4");
     const int old_mode = c->mode_req;
     const uint32_t old_fold = c->fold_log2_req;
     c->mode_req = mode;
@@ -1091,9 +1119,10 @@ int ffbf_get_timings(ffbf_ctx *c, ffbf_timings *out)
     out->verify_ms = elapsed(c, EV_V0, EV_V1);
     out->scan_mode = c->mode;
     out->fold_log2_bits = c->mode == ffbf::MODE_GLOBAL ? 0u : pow2_at_least((uint64_t)c->pre_words * 32u);
-    out->probes = (c->mode == ffbf::MODE_FOLD || c->mode == ffbf::MODE_LDS) ? std::min(c->probes, 3u) : 1u;
+    out->probes = ffbf::mode_in_lds(c->mode) ? std::min(c->probes, 3u) : 1u;
     out->block = scan_block(c);
     out->ingest_ms = (float)c->ingest_ms;
+    out->l2_fold_log2_bits = c->gfold_words ? pow2_at_least((uint64_t)c->gfold_words * 32u) : 0u;
     return FFBF_OK;
 }
 

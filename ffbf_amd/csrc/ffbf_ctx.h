@@ -139,6 +139,7 @@ struct ffbf_ctx {
     uint32_t probes_req = 0, block_req = 0;
     uint32_t mode = ffbf::MODE_GLOBAL;
     uint32_t pre_words = 0;   // prefilter size in words (F itself, LDS fold or global fold)
+    uint32_t gfold_words = 0; // size of the L2-resident fold behind the LDS fold (mode 4)
     uint32_t probes = 1;      // LDS prefilter bits tested before queueing
     double pre_fill = 0.0;    // expected fraction of prefilter bits set
     uint64_t n_long = 0;      // patterns with length >= W (each sets k bits of F)

@@ -43,7 +43,12 @@ enum ScanMode : uint32_t {
     MODE_FOLD = 1,    // OR-fold of F held in LDS tested first (exact superset), then F
     MODE_LDS = 2,     // the whole of F held in LDS: the LDS test is exact
     MODE_GFOLD = 3,   // OR-fold of F in global memory, small enough to stay in L2, then F
+    MODE_HYBRID = 4,  // LDS fold first, then the L2-resident fold's first bit before queueing;
+                      // the queue drains through the L2 fold's other bits, then F
 };
+
+// Modes whose scan holds a prefilter in each workgroup's LDS.
+__host__ __device__ inline bool mode_in_lds(uint32_t m) { return m == MODE_FOLD || m == MODE_LDS || m == MODE_HYBRID; }
 
 // Hash method of a context.  PAIR: the GPU reference's two 32-bit rolling
 // hashes and one power-of-two filter (greptest.cu).  SPLIT: the CPU binary's
@@ -89,6 +94,10 @@ struct ScanArgs {
     uint64_t spill_base;    // first entry after all region slots
     uint32_t method;        // HashMethod; SPLIT: sbox is 256 (lo, hi) pairs, mask unused
     SplitMap split;
+    // MODE_HYBRID: the L2-resident fold behind the LDS fold (f_first tests its
+    // first bit instead of F's; the drain tests its other bits, then all of F)
+    const uint32_t *pre2;
+    uint32_t pre2_mask;
 };
 
 // Batched scan: many texts (concatenated in one device buffer), each routed

@@ -185,7 +185,7 @@ hipError_t launch_scan_split_w(const ScanArgs &a, uint32_t mode, int n_cu, hipSt
 template <int WT>
 hipError_t launch_scan_w(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s)
 {
-    const bool lds = mode == MODE_FOLD || mode == MODE_LDS;
+    const bool lds = mode_in_lds(mode);
     if (lds) {
         const uint32_t P = a.P < 1u ? 1u : (a.P > 3u ? 3u : a.P);
         if (block == 512) {

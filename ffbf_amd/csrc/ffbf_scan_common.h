@@ -453,6 +453,24 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
         for (int j = P; j < 5; j++) bits[j] = lds_bit(s_pre, fwd_idx(j, h1, h2), a.pre_log2w);
 #pragma unroll
         for (int j = P; j < 5; j++) ok &= ((uint32_t)j < a.k) ? bits[j] : 1u;
+        if (a.pre2) {
+            // the L2-resident fold's bits 1..k-1 (bit 0 was tested before
+            // queueing): bit 1 for all, the rest only where it is set
+            if (ok && a.k > 1u) {
+                const uint32_t i1 = fwd_idx(1, h1, h2) & a.pre2_mask;
+                ok &= (a.pre2[i1 >> 5] >> (i1 & 31u)) & 1u;
+            }
+            if (ok && a.k > 2u) {
+                uint32_t words[5], idx[5];
+#pragma unroll
+                for (int j = 2; j < 5; j++) {
+                    idx[j] = fwd_idx(j, h1, h2) & a.pre2_mask;
+                    words[j] = ((uint32_t)j < a.k) ? a.pre2[idx[j] >> 5] : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (int j = 2; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
+            }
+        }
     } else {
         // the global prefilter tested bit 0 only; bit 1 next, then the rest
         // only for the few lanes still passing (each step is one round trip
@@ -477,7 +495,7 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
         // only where that bit is set.  F is too big for L2 whenever a fold is
         // in use, and its fill is low, so this takes ~1 random access per
         // window instead of k
-        if (ok && !(LDSPRE && a.f_first)) {  // with f_first the bit was tested before queueing
+        if (ok && !(LDSPRE && a.f_first && !a.pre2)) {  // F's own first bit may have been tested before queueing
             const uint32_t i0 = h1 & a.mask;
             ok &= (a.F[i0 >> 5] >> (i0 & 31u)) & 1u;
         }
@@ -547,12 +565,15 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, uint32_
     }
     pm &= vm;
     if constexpr (LDSPRE && !SPLIT) {
-        // a fold that passes many windows: F's first forward bit (h1) now,
-        // so that only windows passing it are queued
+        // a fold that passes many windows: the first forward bit (h1) of F,
+        // or of the L2-resident fold behind it, now, so that only windows
+        // passing it are queued
         if (a.f_first && pm) {
+            const uint32_t *src = a.pre2 ? a.pre2 : a.F;
+            const uint32_t smask = a.pre2 ? a.pre2_mask : a.mask;
             uint32_t fw[kUnit];
 #pragma unroll
-            for (int b = 0; b < kUnit; b++) fw[b] = ((pm >> b) & 1u) ? a.F[(h1[b] & a.mask) >> 5] : 0xFFFFFFFFu;
+            for (int b = 0; b < kUnit; b++) fw[b] = ((pm >> b) & 1u) ? src[(h1[b] & smask) >> 5] : 0xFFFFFFFFu;
             uint32_t keep = 0;
 #pragma unroll
             for (int b = 0; b < kUnit; b++) keep |= ((fw[b] >> (h1[b] & 31u)) & 1u) << b;

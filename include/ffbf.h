@@ -100,12 +100,14 @@ typedef struct ffbf_timings {
     float scan_kernel_ms;  /* the scan kernel alone */
     float pattern_pass_ms; /* pattern pass (+ survivor index) */
     float verify_ms;       /* exact verification */
-    uint32_t scan_mode;      /* 0 F probed in HBM/L2, 1 LDS fold, 2 whole F in LDS, 3 L2-resident fold */
+    uint32_t scan_mode;      /* 0 F probed in HBM/L2, 1 LDS fold, 2 whole F in LDS, 3 L2-resident fold,
+                                4 LDS fold then L2-resident fold */
     uint32_t fold_log2_bits; /* prefilter size (0 in mode 0) */
     uint32_t probes;         /* forward bits tested against the prefilter before queueing */
     uint32_t block;          /* scan workgroup size */
     float ingest_ms;         /* last scan of host memory or a file: first byte read to last
                                 candidate recorded (host clock, includes the overlapped copies) */
+    uint32_t l2_fold_log2_bits; /* modes 3 and 4: size of the L2-resident fold (else 0) */
 } ffbf_timings;
 
 typedef struct ffbf_ctx ffbf_ctx;
@@ -262,7 +264,9 @@ int ffbf_filter_load(ffbf_ctx *ctx, const uint32_t *deltas, uint64_t n);
  * mode -1 picks automatically after each build; 0 first probe in F;
  * 1 OR-fold of F held in LDS, fold_log2_bits bits (0 = largest that fits);
  * 2 whole filter in LDS (only when it fits); 3 OR-fold of F in global memory
- * sized to stay in L2 (fold_log2_bits, 0 = 2^24).  A fold bit is the OR of
+ * sized to stay in L2 (fold_log2_bits, 0 = the largest that fits one XCD's
+ * L2, 2^25 bits on MI355X); 4 (F larger than L2) the LDS fold of mode 1
+ * (fold_log2_bits), then mode 3's fold, then F.  A fold bit is the OR of
  * all F bits congruent to it, so the fold never rejects a true candidate.
  * Takes effect at the next scan. */
 int ffbf_set_scan_mode(ffbf_ctx *ctx, int mode, uint32_t fold_log2_bits);

@@ -3,7 +3,7 @@
 // executor checks the admission rules (slot bytes, pieces per batch, HBM
 // budget in flight) and can fail batches or hold them until released.
 // Built with g++ under AddressSanitizer + UndefinedBehaviorSanitizer, and
-// under ThreadSanitizer, by tests/test_sched.py.  Exits 0 when every check
+// under ThreadSanitizer, by tests/test_sanitize.py.  Exits 0 when every check
 // passes.
 #include <atomic>
 #include <cstdio>

@@ -76,6 +76,7 @@ SPECS = [
     (3000, 22, fa.ALPHA_ASCII, fa.MODE_FOLD),        # LDS fold
     (20000, 18, fa.ALPHA_DNA_GC40, fa.MODE_GLOBAL),  # F probed in global memory
     (60000, 26, fa.ALPHA_ASCII, fa.MODE_GFOLD),      # L2-sized global fold
+    (200000, 27, fa.ALPHA_DNA_GC40, fa.MODE_HYBRID),  # LDS fold, then the L2-sized fold
 ]
 
 

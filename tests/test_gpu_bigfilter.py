@@ -27,7 +27,7 @@ def test_large_filters_match_oracle(log2):
         m.set_patterns(pb, po)
         m.build()
         assert np.array_equal(m.filter(fa.FILTER_FORWARD), ref["F"])
-        for mode in (fa.MODE_AUTO, fa.MODE_GLOBAL, fa.MODE_FOLD, fa.MODE_GFOLD):
+        for mode in (fa.MODE_AUTO, fa.MODE_GLOBAL, fa.MODE_FOLD, fa.MODE_GFOLD, fa.MODE_HYBRID):
             m.set_scan_mode(mode)
             m.run_device(dev.ptr, text.size)
             got = m.matches()

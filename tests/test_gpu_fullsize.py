@@ -92,7 +92,8 @@ def test_full_size_1m_dna_patterns_2gib_text():
         m.set_patterns(pb, po)
         m.build()
         assert np.array_equal(m.filter(fa.FILTER_FORWARD), r["F"]), "forward filter differs"
-        for mode in (fa.MODE_AUTO, fa.MODE_GLOBAL, fa.MODE_FOLD):  # the LDS fold passes ~93%: F tested before queueing
+        # the LDS fold passes ~93% of windows: F (mode 1) or the L2 fold (mode 4) tested before queueing
+        for mode in (fa.MODE_AUTO, fa.MODE_GLOBAL, fa.MODE_FOLD, fa.MODE_GFOLD, fa.MODE_HYBRID):
             m.set_scan_mode(mode)
             say(f"  mode request {mode}: scan mode {m.timings()['scan_mode']}")
             assert_same(gpu_staged(m, dev, N_TEXT), r)
@@ -160,7 +161,7 @@ def test_full_size_100k_ascii_patterns_1536mib_text():
         m.set_patterns(pb, po)
         m.build()
         assert np.array_equal(m.filter(fa.FILTER_FORWARD), r["F"]), "forward filter differs"
-        for mode in (fa.MODE_AUTO, fa.MODE_GFOLD, fa.MODE_GLOBAL):
+        for mode in (fa.MODE_AUTO, fa.MODE_GFOLD, fa.MODE_HYBRID, fa.MODE_GLOBAL):
             m.set_scan_mode(mode)
             say(f"  mode request {mode}: scan mode {m.timings()['scan_mode']}")
             assert_same(gpu_staged(m, dev, N_TEXT), r)

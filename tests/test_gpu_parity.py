@@ -95,6 +95,11 @@ RANDOM_CASES = [
     (fa.ALPHA_DNA_GC40, 32, 4, 22, 20000, 32, 100, 3_000_000, 500, fa.MODE_GFOLD),
     (fa.ALPHA_ASCII, 24, 3, 20, 5000, 24, 80, 1_000_000, 300, fa.MODE_GFOLD),
     (fa.ALPHA_ASCII, 13, 4, 18, 5000, 13, 40, 1_000_000, 300, fa.MODE_GFOLD),     # generic W
+    # LDS fold, then the L2 fold: F beyond L2 (2^26 bits and up)
+    (fa.ALPHA_DNA_GC40, 32, 4, 26, 400_000, 32, 100, 2_000_000, 500, fa.MODE_HYBRID),
+    (fa.ALPHA_ASCII, 19, 5, 27, 50_000, 19, 64, 1_000_000, 300, fa.MODE_HYBRID),   # k=5
+    (fa.ALPHA_ASCII, 13, 3, 26, 20_000, 13, 40, 1_000_000, 300, fa.MODE_HYBRID),   # generic W, k=3
+    (fa.ALPHA_DNA_GC40, 24, 2, 26, 300_000, 24, 60, 1_000_000, 300, fa.MODE_HYBRID),  # k=2
 ]
 
 

@@ -217,7 +217,7 @@ def test_split_rejected_where_unsupported():
     sb = sbox64(16)
     b, off = fa.gen_patterns(fa.ALPHA_ASCII, 17, 100, 19, 30)
     with make_split(19, 5, 2, 16, 18, sb, b, off) as m:
-        for mode in (fa.MODE_LDS, fa.MODE_GFOLD):
+        for mode in (fa.MODE_LDS, fa.MODE_GFOLD, fa.MODE_HYBRID):
             with pytest.raises(fa.FFBFError) as e:
                 m.set_scan_mode(mode)
             assert e.value.status == fa.EINVAL
