@@ -509,8 +509,10 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
     array of the prefilter's size measured now.  Probes per window follow
     from the prefilter's fill f: bit 0 for every window, bit 1 where bit 0 is
     set, bits 2..k-1 together where both are (F's own bits after a full fold
-    hit add ~f^k); behind an LDS fold of fill l, only the share l^P of
-    windows passing its P up-front bits probe at all."""
+    hit add ~f^k).  Behind an LDS fold of fill l only some windows probe: the
+    share l^P passing its P up-front bits when that exceeds 5% (they probe
+    before queueing), else the share l^k passing all its k bits (they probe
+    in the drain)."""
     import numpy as np
 
     t = m.timings()
@@ -528,7 +530,10 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
     pre_log2 = t["l2_fold_log2_bits"] if t["scan_mode"] in (3, 4) else (F.size * 32).bit_length() - 1
     pre = fold(pre_log2) if t["scan_mode"] in (3, 4) else F
     f = fill(pre)
-    lds_pass = fill(fold(t["fold_log2_bits"])) ** t["probes"] if t["scan_mode"] == 4 else 1.0
+    lds_pass = 1.0
+    if t["scan_mode"] == 4:
+        lf = fill(fold(t["fold_log2_bits"]))
+        lds_pass = lf ** t["probes"] if lf ** t["probes"] > 0.05 else lf ** k
     per_window = lds_pass * (1.0 + (f if k > 1 else 0.0) + (max(0, k - 2) * f * f))
     windows = max(0, n_text - W + 1)
     rate = windows * per_window / (t_scan_ms * 1e-3)
@@ -538,7 +543,7 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
                       4: "L2-resident fold of F behind an LDS fold"}[t["scan_mode"]],
         "prefilter_bytes": pre.size * 4,
         "prefilter_fill": round(f, 5),
-        "lds_pass": round(lds_pass, 4),
+        "lds_pass": round(lds_pass, 5),
         "probes_per_window": round(per_window, 4),
         "scan_probes_per_s": round(rate, 1),
         "gather_probes_per_s": round(ceiling, 1),
@@ -861,7 +866,9 @@ def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
             "kernel": "k_scan",
             "bound": "hbm",
             # what holds the kernel below that roof, from its L2 and SQ counters
-            "limiter": ("L2 random-request rate" if tm["scan_mode"] in (0, 3, 4) else "VALU/LDS instruction issue"),
+            "limiter": ("L2 random-request rate"
+                        if tm["scan_mode"] in (0, 3) or (ceiling and ceiling["lds_pass"] > 0.05)
+                        else "VALU/LDS instruction issue"),
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -156,9 +156,9 @@ int refresh_filters(ffbf_batch *b)
             b->cstage.fill(a, half, b->cstage.n_regions);
         a.counters = b->counters.p;
         a.P = lds ? b->probes_eff : 1u;
-        a.f_first = (c->mode == MODE_HYBRID || (c->mode == MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)))
-                        ? 1u
-                        : 0u;
+        a.f_first =
+            ((c->mode == MODE_FOLD || c->mode == MODE_HYBRID) && !c->split && fold_passes_many(c->pre_fill, a.P)) ? 1u
+                                                                                                                : 0u;
         VerifyArgs &v = va[f];
         v = VerifyArgs{};
         v.bytes = c->bytes.p;

@@ -240,9 +240,12 @@ int choose_mode(ffbf_ctx *c)
                 gl2 = l2_fold_log2(c);
                 const double g = fold_fill(c, 1ull << gl2);
                 cost_gfold = (1.0 + g + g * g) * kL2 + std::pow(g, (double)k) * kFar;
-                // the LDS stage overlaps the L2 probes of the windows it
-                // lets through, whichever of the two takes longer
-                cost_hybrid = std::max(kLds, pass * cost_gfold);
+                // an LDS fold passing many windows: the LDS stage overlaps the
+                // L2 probes of the windows it lets through, whichever of the
+                // two takes longer; else the windows passing all its bits
+                // probe the L2 fold in the drain instead of F
+                cost_hybrid = early ? std::max(kLds, pass * cost_gfold)
+                                    : kLds + std::pow(ff, (double)k) * cost_gfold + pass * kQueue;
             }
             if (cost_global < cost_fold && cost_global <= cost_gfold && cost_global <= cost_hybrid)
                 choice = ffbf::MODE_GLOBAL;
@@ -369,11 +372,12 @@ void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
     a.lds = ffbf::mode_in_lds(c->mode) ? 1u : 0u;
     a.f_first = (c->mode == ffbf::MODE_FOLD && !c->split && fold_passes_many(c->pre_fill, a.P)) ? 1u : 0u;
     if (c->mode == ffbf::MODE_HYBRID) {
-        // the L2-resident fold's first bit before queueing, its other bits
-        // in the drain, then F
+        // the L2-resident fold, then F, behind the LDS fold: its first bit
+        // before queueing when the LDS fold passes many windows, the rest in
+        // the drain
         a.pre2 = c->gfold.p;
         a.pre2_mask = (uint32_t)((uint64_t)c->gfold_words * 32u - 1u);
-        a.f_first = 1u;
+        a.f_first = fold_passes_many(c->pre_fill, a.P) ? 1u : 0u;
     }
     // whole F in LDS and small: R is gathered beside it in LDS.  Below 2^17
     // bits the candidates' atomics crowd onto a few cache lines of a global

@@ -454,21 +454,23 @@ static __device__ __forceinline__ void drain(const uint4 *q, uint32_t cnt, const
 #pragma unroll
         for (int j = P; j < 5; j++) ok &= ((uint32_t)j < a.k) ? bits[j] : 1u;
         if (a.pre2) {
-            // the L2-resident fold's bits 1..k-1 (bit 0 was tested before
-            // queueing): bit 1 for all, the rest only where it is set
-            if (ok && a.k > 1u) {
-                const uint32_t i1 = fwd_idx(1, h1, h2) & a.pre2_mask;
-                ok &= (a.pre2[i1 >> 5] >> (i1 & 31u)) & 1u;
+            // the L2-resident fold: its first untested bit j0 for every window
+            // still passing (bit 0 was tested before queueing when f_first is
+            // set), the rest only where that one is set
+            const uint32_t j0 = a.f_first ? 1u : 0u;
+            if (ok && j0 < a.k) {
+                const uint32_t i0 = fwd_idx((int)j0, h1, h2) & a.pre2_mask;
+                ok &= (a.pre2[i0 >> 5] >> (i0 & 31u)) & 1u;
             }
-            if (ok && a.k > 2u) {
+            if (ok && a.k > j0 + 1u) {
                 uint32_t words[5], idx[5];
 #pragma unroll
-                for (int j = 2; j < 5; j++) {
+                for (int j = 1; j < 5; j++) {
                     idx[j] = fwd_idx(j, h1, h2) & a.pre2_mask;
-                    words[j] = ((uint32_t)j < a.k) ? a.pre2[idx[j] >> 5] : 0xFFFFFFFFu;
+                    words[j] = ((uint32_t)j > j0 && (uint32_t)j < a.k) ? a.pre2[idx[j] >> 5] : 0xFFFFFFFFu;
                 }
 #pragma unroll
-                for (int j = 2; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
+                for (int j = 1; j < 5; j++) ok &= (words[j] >> (idx[j] & 31u)) & 1u;
             }
         }
     } else {
