@@ -19,7 +19,8 @@
 // to it, so a window whose forward bits are all set in F has them set in the
 // fold: the test never rejects a true candidate).  The prefilter lives in
 // LDS (fold or whole filter) or in global memory (F, or a fold small enough
-// to stay in L2).  Windows that pass are pushed onto a per-wave LDS queue;
+// to stay in L2), or both: an LDS fold in front of the L2-sized fold for a
+// filter larger than L2.  Windows that pass are pushed onto a per-wave LDS queue;
 // 64 at a time, one per lane, they finish their forward probes, feed their
 // reverse bits forward into R and are appended to the candidate list.
 #include "ffbf_scan_common.h"
