@@ -97,6 +97,37 @@ def test_pair_method_random(case):
 
 
 @pytest.mark.parametrize("case", range(N_CASES // 2))
+def test_pair_method_random_beyond_l2(case):
+    """Filters of 2^26-2^27 bits (beyond one XCD's L2) in the modes that
+    probe the L2-resident fold: mode 4 with LDS folds of 2^7-2^20 bits (the
+    small ones pass most windows, so the L2 fold's first bit is tested before
+    queueing; the large ones pass few, and the drain probes it), mode 3, the
+    LDS fold alone and the automatic choice."""
+    rng = np.random.default_rng(9300 + case)
+    W = int(rng.choice([5, 19, 24, 32, 33]))
+    k = int(rng.integers(1, 6))
+    log2 = int(rng.integers(26, 28))
+    mode = int(rng.choice([fa.MODE_AUTO, fa.MODE_FOLD, fa.MODE_GFOLD, fa.MODE_HYBRID, fa.MODE_HYBRID]))
+    fold = {fa.MODE_FOLD: int(rng.integers(7, 21)), fa.MODE_HYBRID: int(rng.integers(7, 21)),
+            fa.MODE_GFOLD: int(rng.integers(20, 26))}.get(mode, 0)
+    probes = int(rng.integers(0, k + 1))
+    block = int(rng.choice([0, 512, 1024]))
+    shift = int(rng.integers(0, 16))
+    sbox = fa.sbox_from_seed(int(rng.integers(1, 1 << 30)))
+    b, off, text = random_inputs(rng, W, lines=False)
+    say(f"pair case beyond L2 {case}: W={W} k={k} log2={log2} mode={mode} fold={fold} probes={probes} "
+        f"block={block} n_pat={off.size - 1} n={text.size} shift={shift}")
+    r = Oracle(W, k, log2, sbox).run(text, b, off)
+    with fa.Matcher(W, k, log2, sbox) as m:
+        m.set_patterns(b, off)
+        m.build()
+        m.set_scan_mode(mode, fold)
+        m.set_scan_tuning(probes, block)
+        assert m.timings()["scan_mode"] == mode or mode == fa.MODE_AUTO
+        check(run_gpu(m, text, shift), r, m.filter(fa.FILTER_FORWARD))
+
+
+@pytest.mark.parametrize("case", range(N_CASES // 2))
 def test_split_method_random(case):
     rng = np.random.default_rng(9500 + case)
     W = int(rng.choice([1, 7, 19, 19, 24, 33, 64]))
