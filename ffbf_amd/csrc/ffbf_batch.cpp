@@ -47,6 +47,11 @@ struct ffbf_batch {
     DevBuf<unsigned long long> ctl;
     DevBuf<unsigned long long> counters;  // C_CAND, C_MATCH
     unsigned long long *h_counters = nullptr;
+    // pinned copy of the match list's first kMatchPrefetch records, made by
+    // every run with its counters: a short list reaches the host with the
+    // run's own synchronisation
+    BatchMatchRec *h_match = nullptr;
+    uint64_t prefetched = 0;  // records of the current list in h_match
     DevBuf<Cand> cand;  // compacted candidate list
     CandStage cstage;   // staging of both scan launches before compaction
     DevBuf<BatchMatchRec> matches;
@@ -99,6 +104,11 @@ uint32_t units_per_lane(const ffbf_batch *b)
 {
     return (takes_32_bytes(b) && b->tuning.bytes_per_lane != 16u) ? 2u : 1u;
 }
+
+// Match records every run copies to the pinned h_match (a captured graph
+// holds this count with the list's address; both change only together with
+// the list, which drops the graph).
+uint64_t prefetch_count(const ffbf_batch *b) { return std::min<uint64_t>(b->matches.cap, kMatchPrefetch); }
 
 uint32_t bits_for(uint64_t v)
 {
@@ -268,6 +278,8 @@ int enqueue_run(ffbf_batch *b, bool timed)
     HIP_TRY(launch_verify_batch(va, b->n_cu, b->stream));
     HIP_TRY(hipMemcpyAsync(b->h_counters, b->counters.p, C_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->stream));
+    HIP_TRY(hipMemcpyAsync(b->h_match, b->matches.p, prefetch_count(b) * sizeof(BatchMatchRec), hipMemcpyDeviceToHost,
+                           b->stream));
     return FFBF_OK;
 }
 
@@ -322,6 +334,7 @@ int sort_on_device(ffbf_batch *b)
     HIP_TRY(sort_batch_matches(b->matches.p, b->n_match, b->sort_keys.p, b->sort_keys.p + b->n_match, b->sort_temp.p, tb,
                                (int)bo, (int)(bo + bi), end_bit, b->stream));
     b->sorted_on_device = true;
+    b->prefetched = 0;  // the records were copied out before the sort reordered them
     return FFBF_OK;
 }
 
@@ -443,6 +456,7 @@ int ffbf_batch_create(ffbf_ctx *const *filters, uint32_t n_filters, ffbf_batch *
         HIP_TRY(hipEventCreate(&b->e_scan0));
         HIP_TRY(hipEventCreate(&b->e_scan1));
         HIP_TRY(hipHostMalloc((void **)&b->h_counters, C_COUNT * sizeof(unsigned long long), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&b->h_match, kMatchPrefetch * sizeof(BatchMatchRec), hipHostMallocDefault));
         RET_IF(b->ctl.ensure(8, "batch control"));
         RET_IF(b->counters.ensure(C_COUNT, "batch counters"));
         RET_IF(b->cand.ensure(1 << 16, "batch candidates"));
@@ -483,6 +497,7 @@ void ffbf_batch_destroy(ffbf_batch *b)
     b->sort_keys.release();
     b->sort_temp.release();
     if (b->h_counters) (void)hipHostFree(b->h_counters);
+    if (b->h_match) (void)hipHostFree(b->h_match);
     for (hipEvent_t e : {b->e_run0, b->e_run1, b->e_scan0, b->e_scan1})
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -612,6 +627,7 @@ int ffbf_batch_run(ffbf_batch *b, uint64_t *n_matches)
     b->n_match = b->h_counters[C_MATCH];
     b->ran = true;
     b->sorted_valid = false;
+    b->prefetched = prefetch_count(b);
     RET_IF(sort_on_device(b));
     if (n_matches) *n_matches = b->n_match;
     return FFBF_OK;
@@ -626,7 +642,9 @@ int ffbf_batch_get_matches(ffbf_batch *b, ffbf_batch_match *out, uint64_t cap, u
     if (b->n_match && !out) return fail(FFBF_EINVAL, "null output");
     if (!b->sorted_valid) {
         b->sorted.resize(b->n_match);
-        if (b->n_match) {
+        if (b->n_match <= b->prefetched) {
+            std::memcpy(b->sorted.data(), b->h_match, b->n_match * sizeof(BatchMatchRec));
+        } else if (b->n_match) {
             HIP_TRY(hipMemcpyAsync(b->sorted.data(), b->matches.p, b->n_match * sizeof(BatchMatchRec),
                                    hipMemcpyDeviceToHost, b->stream));
             HIP_TRY(hipStreamSynchronize(b->stream));
@@ -668,6 +686,7 @@ int ffbf_batch_run_per_text(ffbf_batch *b, uint64_t *n_matches)
     b->sorted = std::move(all);
     b->n_match = b->sorted.size();
     b->sorted_valid = true;
+    b->prefetched = 0;
     b->ran = true;
     if (n_matches) *n_matches = b->n_match;
     return FFBF_OK;

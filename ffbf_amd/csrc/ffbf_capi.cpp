@@ -506,6 +506,7 @@ int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n)
     if (!c->matches.p) RET_IF(c->matches.ensure(65536, "matches"));
     c->scanned = c->passed = c->verified = false;
     c->sorted_valid = false;
+    c->prefetched = 0;
     return FFBF_OK;
 }
 
@@ -527,6 +528,35 @@ int do_scan(ffbf_ctx *c)
 
 // Sorts the verified matches on the device when they pack into 64-bit keys
 // (ids < 2^24, offsets < 2^40); otherwise the host sorts them on copy-out.
+// Grows the pinned landing area to at least n records (geometrically: an
+// allocation of pinned memory is slow).
+int ensure_host_matches(ffbf_ctx *c, uint64_t n)
+{
+    if (n <= c->h_match_cap && c->h_match) return FFBF_OK;
+    const uint64_t want = std::max<uint64_t>({n, 2 * c->h_match_cap, kMatchPrefetch});
+    if (c->h_match) (void)hipHostFree(c->h_match);
+    c->h_match = nullptr;
+    c->h_match_cap = 0;
+    c->prefetched = 0;
+    HIP_TRY(hipHostMalloc((void **)&c->h_match, want * sizeof(MatchRec), hipHostMallocDefault));
+    c->h_match_cap = want;
+    return FFBF_OK;
+}
+
+// Queues a copy of the match list's first kMatchPrefetch records (as many as
+// the list holds room for) to the pinned landing area, after the stages
+// already queued; valid once the stream has reached it.
+int prefetch_matches(ffbf_ctx *c)
+{
+    c->prefetched = 0;
+    const uint64_t k = std::min<uint64_t>(c->matches.cap, kMatchPrefetch);
+    if (!k || !c->matches.p) return FFBF_OK;
+    RET_IF(ensure_host_matches(c, k));
+    HIP_TRY(hipMemcpyAsync(c->h_match, c->matches.p, k * sizeof(MatchRec), hipMemcpyDeviceToHost, c->stream));
+    c->prefetched = k;
+    return FFBF_OK;
+}
+
 int sort_on_device(ffbf_ctx *c)
 {
     c->sorted_on_device = false;
@@ -543,16 +573,19 @@ int sort_on_device(ffbf_ctx *c)
     HIP_TRY(ffbf::sort_matches(c->matches.p, c->n_match, c->sort_keys.p, c->sort_keys.p + c->n_match, c->sort_temp.p,
                                tb, end_bit, c->stream));
     c->sorted_on_device = true;
+    c->prefetched = 0;  // the records were copied out before the sort reordered them
     return FFBF_OK;
 }
 
 int do_verify(ffbf_ctx *c)
 {
     RET_IF(enqueue_verify(c, false));
+    RET_IF(prefetch_matches(c));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
         RET_IF(c->matches.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_MATCH], 2 * c->matches.cap), "matches"));
         RET_IF(enqueue_verify(c, false));
+        RET_IF(prefetch_matches(c));
         RET_IF(sync_counters(c));
     }
     c->n_match = c->h_counters[ffbf::C_MATCH];
@@ -566,9 +599,15 @@ int load_sorted_matches(ffbf_ctx *c)
     if (c->sorted_valid) return FFBF_OK;
     c->sorted.resize(c->n_match);
     if (c->n_match) {
-        HIP_TRY(hipMemcpyAsync(c->sorted.data(), c->matches.p, c->n_match * sizeof(MatchRec), hipMemcpyDeviceToHost,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->n_match > c->prefetched) {
+            // through pinned memory: a copy to pageable memory is staged
+            // and blocks longer
+            RET_IF(ensure_host_matches(c, c->n_match));
+            HIP_TRY(hipMemcpyAsync(c->h_match, c->matches.p, c->n_match * sizeof(MatchRec), hipMemcpyDeviceToHost,
+                                   c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
+        std::memcpy(c->sorted.data(), c->h_match, c->n_match * sizeof(MatchRec));
     }
     if (!c->sorted_on_device)
         std::sort(c->sorted.begin(), c->sorted.end(), [](const MatchRec &a, const MatchRec &b) {
@@ -730,6 +769,7 @@ void ffbf_destroy(ffbf_ctx *c)
     c->q_hits.release();
     c->q_off.release();
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_match) (void)hipHostFree(c->h_match);
     for (int i = 0; i < EV_N; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -907,6 +947,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     RET_IF(enqueue_scan(c));
     RET_IF(enqueue_pass(c, true));
     RET_IF(enqueue_verify(c, true));
+    RET_IF(prefetch_matches(c));
     RET_IF(sync_counters(c));
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
         // rerun the three stages with room for every candidate
@@ -914,6 +955,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
         RET_IF(enqueue_scan(c));
         RET_IF(enqueue_pass(c, true));
         RET_IF(enqueue_verify(c, true));
+        RET_IF(prefetch_matches(c));
         RET_IF(sync_counters(c));
     }
     c->n_cand = c->h_counters[ffbf::C_CAND];
@@ -922,6 +964,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     if (c->h_counters[ffbf::C_MATCH] > c->matches.cap) {
         RET_IF(c->matches.ensure(std::max<uint64_t>(2 * c->h_counters[ffbf::C_MATCH], 2 * c->matches.cap), "matches"));
         RET_IF(enqueue_verify(c, false));
+        RET_IF(prefetch_matches(c));
         RET_IF(sync_counters(c));
     }
     c->n_match = c->h_counters[ffbf::C_MATCH];

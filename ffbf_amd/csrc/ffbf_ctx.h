@@ -172,6 +172,12 @@ struct ffbf_ctx {
     uint64_t n_cand = 0, n_surv = 0, n_match = 0;
     std::vector<ffbf::MatchRec> sorted;
     bool sorted_valid = false;
+    // pinned landing area of the match list on its way to the host: the
+    // fused run copies the first kMatchPrefetch records there before its one
+    // synchronisation, so a short list needs no second round trip
+    ffbf::MatchRec *h_match = nullptr;
+    uint64_t h_match_cap = 0;  // records
+    uint64_t prefetched = 0;   // leading records of the current list already in h_match
 
     bool patterns = false, built = false, scanned = false, passed = false, verified = false;
     hipEvent_t ev[ffbf::EV_N] = {};

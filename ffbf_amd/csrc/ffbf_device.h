@@ -254,6 +254,8 @@ hipError_t sort_batch_matches(BatchMatchRec *m, uint64_t n, uint64_t *keys_a, ui
 
 // Match lists shorter than this are sorted on the host at copy-out.
 constexpr uint64_t kDeviceSortMin = 4096;
+// Match records a fused run brings to the host speculatively, with its counters.
+constexpr uint64_t kMatchPrefetch = 4096;
 
 // Device sort of matches by (id, offset) through packed 64-bit keys.
 bool sort_packable(uint64_t n_patterns, uint64_t text_bytes);
