@@ -509,10 +509,10 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
     array of the prefilter's size measured now.  Probes per window follow
     from the prefilter's fill f: bit 0 for every window, bit 1 where bit 0 is
     set, bits 2..k-1 together where both are (F's own bits after a full fold
-    hit add ~f^k).  Behind an LDS fold of fill l only some windows probe: the
-    share l^P passing its P up-front bits when that exceeds 5% (they probe
-    before queueing), else the share l^k passing all its k bits (they probe
-    in the drain)."""
+    hit add ~f^k).  Behind an LDS fold of fill l only some windows probe: when
+    its P up-front bits pass more than 5%, the share l^(P+1) passing those and
+    one more LDS bit (they probe before queueing), else the share l^k passing
+    all its k bits (they probe in the drain)."""
     import numpy as np
 
     t = m.timings()
@@ -533,7 +533,7 @@ def probe_ceiling(fa, m, k, n_text, W, t_scan_ms):
     lds_pass = 1.0
     if t["scan_mode"] == 4:
         lf = fill(fold(t["fold_log2_bits"]))
-        lds_pass = lf ** t["probes"] if lf ** t["probes"] > 0.05 else lf ** k
+        lds_pass = lf ** min(t["probes"] + 1, k) if lf ** t["probes"] > 0.05 else lf ** k
     per_window = lds_pass * (1.0 + (f if k > 1 else 0.0) + (max(0, k - 2) * f * f))
     windows = max(0, n_text - W + 1)
     rate = windows * per_window / (t_scan_ms * 1e-3)
@@ -867,7 +867,7 @@ def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
             "bound": "hbm",
             # what holds the kernel below that roof, from its L2 and SQ counters
             "limiter": ("L2 random-request rate"
-                        if tm["scan_mode"] in (0, 3) or (ceiling and ceiling["lds_pass"] > 0.05)
+                        if tm["scan_mode"] in (0, 3) or (ceiling and ceiling["lds_pass"] > 0.04)
                         else "VALU/LDS instruction issue"),
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,

@@ -571,6 +571,15 @@ static __device__ __forceinline__ void test_and_queue(const ScanArgs &a, uint32_
         // or of the L2-resident fold behind it, now, so that only windows
         // passing it are queued
         if (a.f_first && pm) {
+            if (a.pre2 && (uint32_t)P < a.k) {
+                // the LDS fold's next bit first: each window it rejects saves
+                // an L2 request, and L2 requests bound this mode (the drain
+                // tests the bit again, to the same result)
+                uint32_t nb = 0;
+#pragma unroll
+                for (int b = 0; b < kUnit; b++) nb |= lds_bit(s_pre, fwd_idx(P, h1[b], h2[b]), a.pre_log2w) << b;
+                pm &= nb;
+            }
             const uint32_t *src = a.pre2 ? a.pre2 : a.F;
             const uint32_t smask = a.pre2 ? a.pre2_mask : a.mask;
             uint32_t fw[kUnit];
