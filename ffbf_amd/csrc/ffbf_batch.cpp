@@ -219,7 +219,6 @@ int refresh_filters(ffbf_batch *b)
     for (ffbf_ctx *c : b->filters) {
         zero(c->R.p, c->f_words);
         zero(c->table.p, 2ull * c->table_size);
-        zero((uint32_t *)(c->counters.p + C_SURV), 2u);
     }
     b->n_zero = (uint32_t)zs.size();
     RET_IF(b->d_pass.ensure(nf, "batch pass descriptors"));

@@ -261,17 +261,26 @@ int choose_mode(ffbf_ctx *c)
         }
         if (choice == ffbf::MODE_FOLD || choice == ffbf::MODE_HYBRID) {
             RET_IF(c->fold.ensure(words, "fold"));
-            HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
-            mode = ffbf::MODE_FOLD;
-            c->pre_words = words;
-            c->pre_fill = ff;
             if (choice == ffbf::MODE_HYBRID) {
+                // the L2 fold first; the LDS fold is then the fold of the L2
+                // fold (both sizes are powers of two, so folding onto 2^a
+                // bits after 2^b > 2^a bits gives the same bits), which
+                // reads 4 MiB instead of all of F
                 const uint32_t gw = 1u << (gl2 - 5);
                 RET_IF(c->gfold.ensure(gw, "global fold"));
                 HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, gw, c->stream));
                 c->gfold_words = gw;
+                if (gw > words)
+                    HIP_TRY(ffbf::launch_fold(c->gfold.p, gw, c->fold.p, words, c->stream));
+                else
+                    HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
                 mode = ffbf::MODE_HYBRID;
+            } else {
+                HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
+                mode = ffbf::MODE_FOLD;
             }
+            c->pre_words = words;
+            c->pre_fill = ff;
         } else if (choice == ffbf::MODE_GLOBAL) {
             mode = ffbf::MODE_GLOBAL;
             c->pre_words = (uint32_t)c->f_words;

@@ -126,7 +126,40 @@ __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const
 __global__ void k_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < fold_words; j += stride) {
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((fold_words & 3u) == 0u && f_words % fold_words == 0u && (((uintptr_t)F | (uintptr_t)fold) & 15u) == 0u) {
+        // 16-byte columns: a thread ORs the r = f_words / fold_words
+        // congruent 16-byte pieces of its column, eight loads in flight at a
+        // time (the fold of a 2^23-word F onto 2^15 words has r = 256)
+        const uint64_t n4 = fold_words / 4u, r = f_words / fold_words;
+        const uint4 *F4 = (const uint4 *)F;
+        for (uint64_t j = t0; j < n4; j += stride) {
+            uint32_t x = 0, y = 0, z = 0, w = 0;
+            uint64_t t = 0;
+            for (; t + 8u <= r; t += 8u) {
+                uint4 v[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) v[i] = F4[j + (t + (uint64_t)i) * n4];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    x |= v[i].x;
+                    y |= v[i].y;
+                    z |= v[i].z;
+                    w |= v[i].w;
+                }
+            }
+            for (; t < r; t++) {
+                const uint4 v = F4[j + t * n4];
+                x |= v.x;
+                y |= v.y;
+                z |= v.z;
+                w |= v.w;
+            }
+            ((uint4 *)fold)[j] = make_uint4(x, y, z, w);
+        }
+        return;
+    }
+    for (uint64_t j = t0; j < fold_words; j += stride) {
         uint32_t v = 0;
         for (uint64_t t = j; t < f_words; t += fold_words) v |= F[t];
         fold[j] = v;
@@ -199,19 +232,29 @@ __device__ __forceinline__ uint32_t pass_wave(const PassArgs &a, uint64_t i0)
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
 {
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
     const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint32_t n = 0;
     for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) n += pass_wave<SPLIT>(a, i0);
-    // one count per wave: a counter shared by every wave-iteration of the
-    // launch would serialize thousands of atomics on one address
-    if (n && lane_id() == 0) atomicAdd(a.counters + C_SURV, (unsigned long long)n);
+    // one count per block: atomics on one global address serialise, and a
+    // per-wave add costs thousands of them per launch
+    if (n && lane_id() == 0) atomicAdd(&s_n, n);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) atomicAdd(a.counters + C_SURV, (unsigned long long)s_n);
 }
 
 // The pattern passes of several filters in one launch: wave_start[f] is the
 // first global 64-id wave of filter f (prefix over ceil(n_f/64)).  A filter
 // whose `active` flag is clear (no text of the batch routed to it) has no
-// survivors: its bitmap words are cleared and nothing is indexed.
+// survivors: its bitmap words are cleared and nothing is indexed.  Waves
+// sweep the ids in order (grid stride), so the filters whose R is probed at
+// any moment are few and stay in L2.  Survivors are not counted: a batch
+// reports matches, and each filter's bitmap holds its survivors (a count per
+// wave, added on one counter per filter, costs the largest filter thousands
+// of serialised atomics).
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filters, const uint64_t *wave_start,
                                                             uint32_t nf, const uint32_t *active)
@@ -229,8 +272,7 @@ __global__ __launch_bounds__(256) void k_pattern_pass_multi(const PassArgs *filt
         const PassArgs &a = filters[lo];
         const uint64_t i0 = (gw - wave_start[lo]) << 6;
         if (active[lo]) {
-            const uint32_t n = pass_wave<SPLIT>(a, i0);
-            if (n && lane_id() == 0) atomicAdd(a.counters + C_SURV, (unsigned long long)n);
+            (void)pass_wave<SPLIT>(a, i0);
         } else {
             const uint64_t n_words = (a.n + 31u) / 32u, w = i0 >> 5;
             const int lane = lane_id();
@@ -374,7 +416,10 @@ hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_
 
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s)
 {
-    uint64_t blocks = ((uint64_t)fold_words + 255u) / 256u;
+    // one thread per 16-byte column of the fold (per word for folds of fewer
+    // than 4 words)
+    const uint64_t cols = (fold_words & 3u) == 0u ? fold_words / 4u : fold_words;
+    uint64_t blocks = (cols + 255u) / 256u;
     if (blocks < 1u) blocks = 1u;
     hipLaunchKernelGGL(k_fold, dim3((uint32_t)blocks), dim3(256), 0, s, F, f_words, fold, fold_words);
     return hipGetLastError();
