@@ -807,7 +807,84 @@ def traffic_from_profiles(name):
         return None
 
 
-def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
+PMC_COPY_BYTES = 1 << 30  # the copy kernel whose counters scale the scan's
+
+
+def measure_traffic(name, timeout_s=150):
+    """HBM bytes per k_scan launch of workload `name`, measured now: two
+    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over a child run of
+    the same workload (one warmup and two timed launches) followed by three
+    launches of the library's copy kernel over PMC_COPY_BYTES.  Each counter is
+    scaled by what it reads for the copy, which reads and writes exactly
+    PMC_COPY_BYTES.  None when rocprofv3 is missing, when this process already
+    runs under a profiler (a nested one would profile itself), or when a pass
+    fails; the caller then reports the committed profile's figure and says so."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    if any(k.startswith("ROCPROF") or k == "ROCP_TOOL_LIBRARIES" for k in os.environ):
+        return None
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="ffbf_pmc_")
+        try:
+            cmd = [prof, "--pmc", counter, "--kernel-include-regex", "k_scan<|k_copy", "-f", "csv", "-d", d, "-o", "p",
+                   "--", sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", "2", "--warmup", "1",
+                   "--traffic-child", str(PMC_COPY_BYTES)]
+            env = dict(os.environ, TMPDIR=tempfile.gettempdir())
+            p = subprocess.Popen(cmd, cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                 start_new_session=True)
+            try:
+                p.wait(timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.wait()
+                log(f"traffic: {counter} pass timed out")
+                return None
+            # rocprofv3 can exit non-zero after a complete run; the CSV decides
+            path = os.path.join(d, "p_counter_collection.csv")
+            if not os.path.exists(path):
+                log(f"traffic: {counter} pass wrote no counters (rc {p.returncode})")
+                return None
+            scan, copy = [], []
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    if "k_scan<" in r["Kernel_Name"]:
+                        scan.append(float(r["Counter_Value"]))
+                    elif "k_copy" in r["Kernel_Name"]:
+                        copy.append(float(r["Counter_Value"]))
+            if len(scan) < 3 or len(copy) < 3 or min(copy) <= 0:
+                log(f"traffic: {counter} pass incomplete ({len(scan)} scan, {len(copy)} copy dispatches)")
+                return None
+            scan.sort()
+            copy.sort()
+            got[counter] = (scan[len(scan) // 2], copy[len(copy) // 2])
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    f_scan, f_copy = got["FETCH_SIZE"]
+    w_scan, w_copy = got["WRITE_SIZE"]
+    f_scale = PMC_COPY_BYTES / (f_copy * 1024.0)
+    w_scale = PMC_COPY_BYTES / (w_copy * 1024.0)
+    return {
+        "bytes": int(round(f_scan * 1024.0 * f_scale + w_scan * 1024.0 * w_scale)),
+        "fetch_kib": round(f_scan, 1),
+        "write_kib": round(w_scan, 1),
+        "fetch_scale": round(f_scale, 4),
+        "write_scale": round(w_scale, 4),
+        "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in this run (median of 3 scan launches), "
+                  f"scaled by the same counters on a {PMC_COPY_BYTES >> 20} MiB copy kernel",
+    }
+
+
+def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True, traffic=None):
     """The single-text path over a device-resident text: each step builds F,
     scans with feed-forward, runs the pattern pass, verifies and brings the
     sorted matches to the host.  Returns the bench fields and the inputs."""
@@ -873,7 +950,10 @@ def run_plain(fa, name, steps, warmup, mode=-1, copy_rate=True):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles(name),
+            "traffic": traffic["bytes"] if traffic else traffic_from_profiles(name),
+            "traffic_source": traffic["source"] if traffic else "committed profile (profiles/scan_traffic.json)",
+            "traffic_counters": ({k: traffic[k] for k in ("fetch_kib", "write_kib", "fetch_scale", "write_scale")}
+                                 if traffic else None),
             "avg_launch_ms": round(t_scan, 4),
             "algorithmic_bytes": n_text,
             "copy_kernel_gbs": copy_kernel_rate(fa) if copy_rate else None,
@@ -912,6 +992,9 @@ def main() -> int:
                     help="only run the random-gather microbenchmark (4-byte probes into 16 KiB..2 GiB) and exit")
     ap.add_argument("--gather-log2", type=int, default=0, metavar="L",
                     help="only run the random-gather microbenchmark into a 2^L-byte array (two launches) and exit")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure the scan's HBM traffic")
+    ap.add_argument("--traffic-child", type=int, default=0, metavar="BYTES", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -921,9 +1004,29 @@ def main() -> int:
                           "n_gpus": args.gpus}))
         return 2
 
+    # the scan's HBM traffic, from counters of child runs, before this process
+    # touches the GPU
+    traffic = None
+    if (not args.traffic_child and not args.no_traffic and args.workload in WORKLOADS and
+            not (args.copy_calibration or args.gather_log2 or args.gather_sweep)):
+        log("traffic: two rocprofv3 --pmc passes ...")
+        traffic = measure_traffic(args.workload)
+
     import numpy as np
 
     import ffbf_amd as fa
+
+    if args.traffic_child:
+        # a profiled child of measure_traffic: the workload's scans, then the
+        # copy kernel whose counters scale theirs
+        run_plain(fa, args.workload, args.steps, args.warmup, args.mode, copy_rate=False)
+        n = args.traffic_child & ~15
+        src, dst = fa.DeviceBuffer(n), fa.DeviceBuffer(n)
+        for _ in range(3):
+            dst.copy_from(src, n)
+        src.free()
+        dst.free()
+        return 0
 
     if args.copy_calibration:
         n = args.copy_calibration & ~15
@@ -985,7 +1088,7 @@ def main() -> int:
                           "batched": r}), flush=True)
         return 0
 
-    out, inputs = run_plain(fa, args.workload, args.steps, args.warmup, args.mode)
+    out, inputs = run_plain(fa, args.workload, args.steps, args.warmup, args.mode, traffic=traffic)
     out = {"metric": METRIC, **out}
     if args.workload == "plain" and not args.no_batched:
         log("config 1 leg (ascii10k-64m) ...")
