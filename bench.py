@@ -262,7 +262,7 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
-def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20):
+def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20, reps=5):
     """Continuous-batching server vs the plain per-request path (one
     ffbf_match_text per request, first come first served) on the same
     requests.  Saturation throughput of each (everything queued at once);
@@ -270,7 +270,8 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     submitting thread: at loads stated against the plain path's own
     saturation rate (where both are stable), and at loads against the
     server's (where the plain path is past its saturation, so it is reported
-    as unstable and not driven)."""
+    as unstable and not driven).  Each load point is `reps` passes over the
+    same arrivals."""
     import queue
     import threading
 
@@ -309,14 +310,19 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     # saturation: everything submitted at once; the timed region ends when
     # the last request's matches are in hand (compared with the plain path's
     # afterwards)
-    t1 = time.perf_counter()
-    ids = srv.submit_many(reqs)
-    got_all = [srv.wait(rid)[0] for rid in ids]
-    sat_s = time.perf_counter() - t1
+    sat_passes = []
+    for r in range(reps):
+        t1 = time.perf_counter()
+        ids = srv.submit_many(reqs)
+        got = [srv.wait(rid)[0] for rid in ids]
+        sat_passes.append(time.perf_counter() - t1)
+        if r == 0:
+            got_all = got
+    sat_s = float(np.median(sat_passes))
     srv_rate = n_requests / sat_s
     same = all(np.array_equal(g["pattern_id"], p["pattern_id"]) and np.array_equal(g["offset"], p["offset"])
                for g, p in zip(got_all, plain_res))
-    got_all = None
+    got_all = got = None
 
     def arrivals_at(lam):
         return np.cumsum(d["gaps"]) / lam
@@ -375,15 +381,28 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     def pct(xs):
         return round(percentile(xs, 50), 3), round(percentile(xs, 99), 3)
 
+    def passes(drive, lam):
+        """`reps` passes over the same seeded arrivals: all their latencies
+        (p50/p99 over reps x n_requests samples, so that one host stall does
+        not make the p99), the mean pass time and each pass's own p99."""
+        lat, el, p99s = [], 0.0, []
+        for _ in range(reps):
+            l1, e1 = drive(lam)
+            lat += l1
+            el += e1
+            p99s.append(round(percentile(l1, 99), 3))
+        return lat, el / reps, p99s
+
     # both paths at the same arrival rates, below the plain path's saturation
     same_rate = []
     for ld in (0.5, 0.8):
         lam = ld * plain_rate
-        s_lat, s_el = drive_server(lam)
-        p_lat, p_el = drive_plain(lam)
+        s_lat, s_el, s_p99s = passes(drive_server, lam)
+        p_lat, p_el, p_p99s = passes(drive_plain, lam)
         row = {"load_vs_plain_saturation": ld, "load_vs_server_saturation": round(lam / srv_rate, 4),
                "rate_per_s": round(lam, 1), "server_p50_ms": pct(s_lat)[0], "server_p99_ms": pct(s_lat)[1],
                "plain_p50_ms": pct(p_lat)[0], "plain_p99_ms": pct(p_lat)[1],
+               "server_p99_ms_per_pass": s_p99s, "plain_p99_ms_per_pass": p_p99s,
                "server_gbs": round(total / s_el / 1e9, 3), "plain_gbs": round(total / p_el / 1e9, 3)}
         same_rate.append(row)
         log(f"serve: same arrivals at {lam:.0f} req/s ({ld} of plain saturation): server p99 {row['server_p99_ms']} ms, "
@@ -393,10 +412,11 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     lat_at_load = None
     for ld in (0.3, 0.5, 0.7, 0.9):
         lam = ld * srv_rate
-        s_lat, s_el = drive_server(lam)
+        s_lat, s_el, s_p99s = passes(drive_server, lam)
         if ld == load:
             lat_at_load, poisson_s = s_lat, s_el
         row = {"load": ld, "rate_per_s": round(lam, 1), "p50_ms": pct(s_lat)[0], "p99_ms": pct(s_lat)[1],
+               "p99_ms_per_pass": s_p99s,
                "gbs": round(total / s_el / 1e9, 3), "load_vs_plain_saturation": round(lam / plain_rate, 3),
                "plain": "unstable: arrivals above its saturation rate" if lam >= plain_rate else "stable"}
         sweep.append(row)
@@ -414,6 +434,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         "text_bytes": int(total),
         "filters": 8,
         "saturation_gbs": round(total / sat_s / 1e9, 3),
+        "saturation_gbs_per_pass": [round(total / x / 1e9, 3) for x in sat_passes],
         "saturation_rate_per_s": round(srv_rate, 1),
         "cold_start_gbs": round(total / cold_s / 1e9, 3),
         "server_create_s": round(create_s, 3),
@@ -433,7 +454,8 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         "slots": slots,
         "max_batch_bytes": max_batch_bytes,
         "latency_drivers": "both paths driven by a submitting thread at seeded Poisson arrivals; latency from "
-                           "submission to matches in hand",
+                           f"submission to matches in hand; percentiles over {reps} passes of the same arrivals",
+        "passes_per_load": reps,
     }
 
 
