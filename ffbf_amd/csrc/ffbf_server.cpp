@@ -141,8 +141,12 @@ int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_
 {
     if (!filters || !n_filters || !out) return fail(FFBF_EINVAL, "need at least one filter");
     *out = nullptr;
-    for (uint32_t f = 0; f < n_filters; f++)
+    for (uint32_t f = 0; f < n_filters; f++) {
         if (!filters[f]) return fail(FFBF_EINVAL, "null filter");
+        // the slots' replicas copy F as built: the sources must have one
+        if (!filters[f]->built || !filters[f]->patterns)
+            return fail(FFBF_ESTATE, "every filter must have patterns and a built filter");
+    }
     std::unique_ptr<ffbf_server> s(new (std::nothrow) ffbf_server());
     if (!s) return fail(FFBF_ENOMEM, "server allocation");
     if (config) s->cfg = *config;

@@ -193,3 +193,25 @@ def test_server_admission_limits():
         assert np.array_equal(got["offset"], expect[0]["offset"])
     for f in filters:
         f[0].close()
+
+
+def test_server_rejects_unbuilt_filter():
+    # the slots copy each filter's F as built: a source with patterns but no
+    # built F is refused at creation
+    say("server unbuilt filter")
+    sbox = fa.sbox_from_seed(94)
+    b, off = fa.gen_patterns(fa.ALPHA_DNA_GC40, 95, 200, W, 40)
+    m = fa.Matcher(W, K, 16, sbox)
+    m.set_patterns(b, off)
+    with pytest.raises(fa.FFBFError) as e:
+        fa.Server([m], hbm_budget_bytes=8 << 20, slots=1)
+    assert e.value.status == fa.ESTATE
+    m.build()
+    with fa.Server([m], hbm_budget_bytes=8 << 20, slots=1) as srv:
+        t = fa.gen_bytes(fa.ALPHA_DNA_GC40, 96, 100_000)
+        fa.plant(97, 20, t, b, off)
+        got, _ = srv.wait(srv.submit(0, t))
+        want = m.match(t).matches
+        assert np.array_equal(got["pattern_id"], want["pattern_id"])
+        assert np.array_equal(got["offset"], want["offset"])
+    m.close()
