@@ -1014,6 +1014,9 @@ def main() -> int:
                     help="only run the random-gather microbenchmark (4-byte probes into 16 KiB..2 GiB) and exit")
     ap.add_argument("--gather-log2", type=int, default=0, metavar="L",
                     help="only run the random-gather microbenchmark into a 2^L-byte array (two launches) and exit")
+    ap.add_argument("--probe-sweep", action="store_true",
+                    help="only run the access-kind microbenchmark (random loads of four cache policies and "
+                         "device-scope atomicOr into 1 MiB..256 MiB; streaming passes over 16 MiB..4 GiB) and exit")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the scan's HBM traffic")
     ap.add_argument("--traffic-child", type=int, default=0, metavar="BYTES", help=argparse.SUPPRESS)
@@ -1030,7 +1033,7 @@ def main() -> int:
     # touches the GPU
     traffic = None
     if (not args.traffic_child and not args.no_traffic and args.workload in WORKLOADS and
-            not (args.copy_calibration or args.gather_log2 or args.gather_sweep)):
+            not (args.copy_calibration or args.gather_log2 or args.gather_sweep or args.probe_sweep)):
         log("traffic: two rocprofv3 --pmc passes ...")
         traffic = measure_traffic(args.workload)
 
@@ -1062,6 +1065,19 @@ def main() -> int:
         ms, rate = fa.gather_bench(args.gather_log2, steps=16)
         print(json.dumps({"gather_array_bytes": 1 << args.gather_log2, "ms": round(ms, 4),
                           "probes_per_s": round(rate, 1)}), flush=True)
+        return 0
+
+    if args.probe_sweep:
+        for kind in (fa.PROBE_LOAD, fa.PROBE_LOAD_NONTEMPORAL, fa.PROBE_LOAD_AGENT, fa.PROBE_LOAD_SYSTEM,
+                     fa.PROBE_ATOMIC_OR):
+            for l in (20, 22, 25, 28):
+                ms, rate = fa.probe_bench(kind, l, steps=16)
+                print(json.dumps({"kind": fa.PROBE_KIND_NAMES[kind], "array_bytes": 1 << l, "ms": round(ms, 4),
+                                  "accesses_per_s": round(rate, 1)}), flush=True)
+        for l in range(24, 33):
+            ms, rate = fa.probe_bench(fa.PROBE_STREAM, l, steps=max(4, 1 << (33 - l)))
+            print(json.dumps({"kind": fa.PROBE_KIND_NAMES[fa.PROBE_STREAM], "array_bytes": 1 << l,
+                              "ms": round(ms, 4), "bytes_per_s": round(rate, 1)}), flush=True)
         return 0
 
     if args.gather_sweep:

@@ -151,6 +151,7 @@ ABI = {
     "ffbf_set_scan_mode": (_i32, [_P, _i32, _u32]),
     "ffbf_set_scan_tuning": (_i32, [_P, _u32, _u32]),
     "ffbf_gather_bench": (_i32, [_i32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
+    "ffbf_probe_bench": (_i32, [_i32, _u32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_get_timings": (_i32, [_P, _c.POINTER(Timings)]),
     "ffbf_synchronize": (_i32, [_P]),
     "ffbf_get_stream": (_P, [_P]),
@@ -222,6 +223,21 @@ def gather_bench(log2_bytes: int, steps: int = 64, device: int = 0):
     ms = _c.c_float(0)
     rate = _c.c_double(0)
     _check(_lib.ffbf_gather_bench(device, log2_bytes, steps, _c.byref(ms), _c.byref(rate)))
+    return ms.value, rate.value
+
+
+PROBE_LOAD, PROBE_LOAD_NONTEMPORAL, PROBE_LOAD_AGENT, PROBE_LOAD_SYSTEM, PROBE_ATOMIC_OR, PROBE_STREAM = range(6)
+PROBE_KIND_NAMES = {PROBE_LOAD: "load", PROBE_LOAD_NONTEMPORAL: "non-temporal load",
+                    PROBE_LOAD_AGENT: "agent-scope atomic load", PROBE_LOAD_SYSTEM: "system-scope atomic load",
+                    PROBE_ATOMIC_OR: "device-scope atomicOr", PROBE_STREAM: "streaming read+write pass"}
+
+
+def probe_bench(kind: int, log2_bytes: int, steps: int = 64, device: int = 0):
+    """(ms, rate) of random accesses of one kind (PROBE_*) into a
+    2^log2_bytes array, in accesses/s; for PROBE_STREAM, `steps` passes that
+    read and write back the whole array, in bytes/s."""
+    ms, rate = _c.c_float(0.0), _c.c_double(0.0)
+    _check(_lib.ffbf_probe_bench(device, log2_bytes, steps, kind, _c.byref(ms), _c.byref(rate)))
     return ms.value, rate.value
 
 

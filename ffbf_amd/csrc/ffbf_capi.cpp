@@ -1264,9 +1264,10 @@ int ffbf_device_count(int *out)
     return FFBF_OK;
 }
 
-int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms, double *probes_per_s)
+int ffbf_probe_bench(int device, uint32_t log2_bytes, uint32_t steps, uint32_t kind, float *ms, double *ops_per_s)
 {
     if (log2_bytes < 4 || log2_bytes > 34 || steps == 0) return fail(FFBF_EINVAL, "log2_bytes must be 4..34, steps > 0");
+    if (kind > FFBF_PROBE_STREAM) return fail(FFBF_EINVAL, "unknown probe kind");
     HIP_TRY(hipSetDevice(device));
     int n_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1285,16 +1286,25 @@ int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return done(fail(FFBF_EDEVICE, "event creation"));
     if (ffbf::launch_fill32(arr.p, 1ull << (log2_bytes - 2), 0x5A5A5A5Au, nullptr) != hipSuccess ||
-        ffbf::launch_gather(arr.p, log2_bytes - 2, steps, out.p, blocks, nullptr) != hipSuccess ||
+        ffbf::launch_gather(arr.p, log2_bytes - 2, steps, out.p, blocks, kind, nullptr) != hipSuccess ||
         hipEventRecord(e0, nullptr) != hipSuccess ||
-        ffbf::launch_gather(arr.p, log2_bytes - 2, steps, out.p, blocks, nullptr) != hipSuccess ||
+        ffbf::launch_gather(arr.p, log2_bytes - 2, steps, out.p, blocks, kind, nullptr) != hipSuccess ||
         hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
-        return done(fail(FFBF_EDEVICE, std::string("gather bench: ") + hipGetErrorString(hipGetLastError())));
+        return done(fail(FFBF_EDEVICE, std::string("probe bench: ") + hipGetErrorString(hipGetLastError())));
     float t = 0.0f;
     (void)hipEventElapsedTime(&t, e0, e1);
     if (ms) *ms = t;
-    if (probes_per_s) *probes_per_s = (double)blocks * 256.0 * 16.0 * (double)steps / ((double)t * 1e-3);
+    // random kinds: accesses per second; the streaming pass: bytes read and
+    // written per second
+    const double ops = kind == FFBF_PROBE_STREAM ? 2.0 * (double)(1ull << log2_bytes) * (double)steps
+                                                 : (double)blocks * 256.0 * 16.0 * (double)steps;
+    if (ops_per_s) *ops_per_s = ops / ((double)t * 1e-3);
     return done(FFBF_OK);
+}
+
+int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms, double *probes_per_s)
+{
+    return ffbf_probe_bench(device, log2_bytes, steps, FFBF_PROBE_LOAD, ms, probes_per_s);
 }
 
 }  // extern "C"

@@ -222,8 +222,9 @@ hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_
                               uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash, hipStream_t s);
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s);
-hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
-                         hipStream_t s);
+// Random-access microbenchmark (kind: FFBF_PROBE_* of ffbf.h).
+hipError_t launch_gather(uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
+                         uint32_t kind, hipStream_t s);
 hipError_t launch_pattern_pass(const PassArgs &a, int n_cu, hipStream_t s);
 // Pattern passes of nf filters in one launch (wave_start[nf+1], device
 // arrays); filters whose active flag is 0 get an empty survivor set.

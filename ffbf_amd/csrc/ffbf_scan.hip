@@ -204,14 +204,44 @@ hipError_t launch_scan_w(const ScanArgs &a, uint32_t mode, uint32_t block, int n
 
 // Random dword gathers into a 2^log2_words-word array: 16 independent loads
 // per lane per step, addresses from a counter hash.
-__global__ __launch_bounds__(256) void k_gather(const uint32_t *arr, uint32_t mask, uint32_t steps, uint32_t *out)
+// Random-access microbenchmark: 16 independent accesses per lane per step at
+// hashed word addresses of arr[0 .. mask]. KIND 0: plain loads; 1:
+// non-temporal loads; 2: relaxed atomic loads at agent scope (they bypass
+// the CU's L1); 3: the same at system scope; 4: device-scope atomicOr with
+// the result unused (the feed-forward's and the build's operation).
+// Streaming pass counterpart: `steps` passes over the whole array, each
+// reading and writing back every 16-byte word (a thread keeps the same words
+// in every pass).
+__global__ __launch_bounds__(256) void k_stream(uint4 *a, uint64_t n16, uint32_t steps)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t s = 0; s < steps; s++)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+            uint4 v = a[i];
+            v.x ^= s + 1u;
+            a[i] = v;
+        }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_gather(uint32_t *arr, uint32_t mask, uint32_t steps, uint32_t *out)
 {
     uint32_t x = mix32((uint32_t)(blockIdx.x * 256u + threadIdx.x) * 0x9E3779B9u + 1u);
     uint32_t acc = 0;
     for (uint32_t s = 0; s < steps; s++) {
         uint32_t v[16];
 #pragma unroll
-        for (int j = 0; j < 16; j++) v[j] = arr[mix32(x + (uint32_t)j * 0x61C88647u) & mask];
+        for (int j = 0; j < 16; j++) {
+            const uint32_t i = mix32(x + (uint32_t)j * 0x61C88647u) & mask;
+            if constexpr (KIND == 0) v[j] = arr[i];
+            else if constexpr (KIND == 1) v[j] = __builtin_nontemporal_load(arr + i);
+            else if constexpr (KIND == 2) v[j] = __hip_atomic_load(arr + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else if constexpr (KIND == 3) v[j] = __hip_atomic_load(arr + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            else {
+                atomicOr(arr + i, 1u << (x & 31u));
+                v[j] = i;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 16; j++) acc ^= v[j];
         x = mix32(x ^ acc);
@@ -251,10 +281,21 @@ hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_c
     }
 }
 
-hipError_t launch_gather(const uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
-                         hipStream_t s)
+hipError_t launch_gather(uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
+                         uint32_t kind, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, arr, (1u << log2_words) - 1u, steps, out);
+    const uint32_t mask = (1u << log2_words) - 1u;
+    switch (kind) {
+    case 0: hipLaunchKernelGGL(k_gather<0>, dim3(blocks), dim3(256), 0, s, arr, mask, steps, out); break;
+    case 1: hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, s, arr, mask, steps, out); break;
+    case 2: hipLaunchKernelGGL(k_gather<2>, dim3(blocks), dim3(256), 0, s, arr, mask, steps, out); break;
+    case 3: hipLaunchKernelGGL(k_gather<3>, dim3(blocks), dim3(256), 0, s, arr, mask, steps, out); break;
+    case 4: hipLaunchKernelGGL(k_gather<4>, dim3(blocks), dim3(256), 0, s, arr, mask, steps, out); break;
+    case 5:
+        hipLaunchKernelGGL(k_stream, dim3(blocks), dim3(256), 0, s, (uint4 *)arr, ((uint64_t)mask + 1u) / 4u, steps);
+        break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -406,6 +406,24 @@ int ffbf_device_copy(void *dst, const void *src, uint64_t bytes);
  * whole device.  Reports the time of one launch and the gather rate. */
 int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms, double *probes_per_s);
 
+/* The same access pattern with another kind of access: plain loads (what
+ * ffbf_gather_bench runs), non-temporal loads, relaxed atomic loads at agent
+ * or system scope, or device-scope atomicOr with the result unused (the
+ * filter build's and the feed-forward's operation); or, for comparison,
+ * `steps` streaming passes that read and write back the whole array.
+ * Reports one launch's time and accesses per second (bytes read and written
+ * per second for the streaming passes).  No counterpart in the reference:
+ * it measures the rates that bound the scan and the build. */
+enum {
+    FFBF_PROBE_LOAD = 0,
+    FFBF_PROBE_LOAD_NONTEMPORAL = 1,
+    FFBF_PROBE_LOAD_AGENT = 2,
+    FFBF_PROBE_LOAD_SYSTEM = 3,
+    FFBF_PROBE_ATOMIC_OR = 4,
+    FFBF_PROBE_STREAM = 5
+};
+int ffbf_probe_bench(int device, uint32_t log2_bytes, uint32_t steps, uint32_t kind, float *ms, double *ops_per_s);
+
 #ifdef __cplusplus
 }
 #endif
