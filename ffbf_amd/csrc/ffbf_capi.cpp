@@ -764,6 +764,8 @@ void ffbf_destroy(ffbf_ctx *c)
     c->bytes.release();
     c->off.release();
     c->pat_hash.release();
+    c->bin_staging.release();
+    c->bin_offs.release();
     c->bitmap.release();
     c->table.release();
     c->chain_next.release();
@@ -854,13 +856,26 @@ int ffbf_build(ffbf_ctx *c)
     RET_IF(check_ctx(c));
     if (!c->patterns) return fail(FFBF_ESTATE, "build before patterns are loaded");
     RET_IF(record(c, EV_B0));
-    HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
-    if (c->split)
-        HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
-                                         c->smap, c->F.p, c->pat_hash.p, c->stream));
-    else
-        HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
-                                   c->mask, c->F.p, c->pat_hash.p, c->stream));
+    const uint64_t slices = (c->f_words + (1u << (ffbf::kBinSliceLog2 - 5)) - 1u) >> (ffbf::kBinSliceLog2 - 5);
+    if (slices >= 2) {
+        // F of several 2^20-bit slices: bits sorted by slice and set in LDS,
+        // each slice written whole (no atomics, no clear), from the prefix
+        // hash pairs ffbf_set_patterns computed
+        const uint64_t nb = ffbf::build_bin_blocks(c->n_pat, c->params.k);
+        RET_IF(c->bin_staging.ensure(std::max<uint64_t>(nb, 1) * ffbf::kBinEntries, "build staging"));
+        RET_IF(c->bin_offs.ensure(std::max<uint64_t>(nb, 1) * (slices + 1u), "build slice offsets"));
+        HIP_TRY(ffbf::launch_build_binned(c->off.p, c->pat_hash.p, c->n_pat, c->params.window, c->params.k, c->mask,
+                                          c->split, c->smap, c->F.p, c->f_words, c->bin_staging.p, c->bin_offs.p,
+                                          c->stream));
+    } else {
+        HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
+        if (c->split)
+            HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
+                                             c->smap, c->F.p, c->pat_hash.p, c->stream));
+        else
+            HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
+                                       c->mask, c->F.p, c->pat_hash.p, c->stream));
+    }
     RET_IF(record(c, EV_B1));
     c->f_set_bits = -1;
     RET_IF(choose_mode(c));

@@ -149,6 +149,9 @@ struct ffbf_ctx {
     ffbf::DevBuf<uint8_t> bytes;
     ffbf::DevBuf<uint64_t> off;
     ffbf::DevBuf<uint2> pat_hash;
+    // the atomic-free build's scratch: bit indices sorted by slice per block,
+    // and each block's slice offsets
+    ffbf::DevBuf<uint32_t> bin_staging, bin_offs;
     ffbf::DevBuf<uint32_t> bitmap, table, chain_next;
     uint64_t n_pat = 0;
     uint32_t table_size = 0;

@@ -220,6 +220,18 @@ hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, c
 // sbox, k bits placed by m (F null: hashes only).
 hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
                               uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash, hipStream_t s);
+// Atomic-free build (filters of more than one 2^20-bit slice): bit indices of
+// kBinEntries / k patterns per block sorted by slice into `staging`
+// (build_bin_blocks(n, k) * kBinEntries words) with per-block slice offsets in
+// `offs` (blocks * (slices + 1) words), then each slice set in LDS and written
+// whole to F.  Reads the prefix hash pairs; F need not be cleared first.
+constexpr uint32_t kBinSliceLog2 = 20;
+constexpr uint32_t kBinEntries = 6144;
+inline uint32_t build_bin_patterns(uint32_t k) { return kBinEntries / (k ? k : 1u); }
+uint64_t build_bin_blocks(uint64_t n, uint32_t k);
+hipError_t launch_build_binned(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W, uint32_t k,
+                               uint32_t mask, bool split, SplitMap m, uint32_t *F, uint64_t f_words, uint32_t *staging,
+                               uint32_t *offs, hipStream_t s);
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
 hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s);
 // Random-access microbenchmark (kind: FFBF_PROBE_* of ffbf.h).

@@ -9,6 +9,8 @@
 //                   (src/rabin-karp/README:49-52)
 //   k_fold          OR-fold of F used by the scan as an exact-superset prefilter
 //   k_build_split   Filter::operator<< of the CPU binary  src/rabin-karp/ffbf.cpp:268-283
+//   k_build_bin, k_build_slices  the same bits for filters of several 2^20-bit
+//                   slices, set without global atomics
 #include "ffbf_common.h"
 
 namespace ffbf {
@@ -117,6 +119,117 @@ __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const
             }
         }
         pat_hash[i] = make_uint2(lo, hi);
+    }
+}
+
+// Atomic-free build of a filter of several 2^20-bit slices.  Device-scope
+// atomics run at a fixed ~27 G/s on this part whatever the array's size, so
+// setting k bits per pattern with atomicOr costs k n / 27e9 s (160 us at 1M
+// patterns and k = 4).  Instead k_build_bin takes kBinEntries / k consecutive
+// patterns per block, forms their forward bit indices from the prefix hash
+// pairs computed when the patterns were set, counting-sorts them by slice in
+// LDS and writes them out in slice order, with the block's slice offsets;
+// k_build_slices then gives each slice one workgroup, which sets that slice's
+// bits from every block in an LDS copy (LDS atomics) and writes the slice
+// whole, clearing what no pattern sets.  Bit placement is the same as
+// k_build's and k_build_split's.
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_build_bin(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t P,
+                                                   uint32_t W, uint32_t k, uint32_t mask, SplitMap m, uint32_t ns,
+                                                   uint32_t *staging, uint32_t *offs)
+{
+    extern __shared__ uint32_t s_bin[];  // raw[kBinEntries], sorted[kBinEntries], hist[ns]
+    uint32_t *raw = s_bin, *sorted = s_bin + kBinEntries, *hist = s_bin + 2u * kBinEntries;
+    __shared__ uint32_t s_ne, s_wave[4];
+    const uint64_t p0 = (uint64_t)blockIdx.x * P;
+    const uint32_t np = (uint32_t)(n - p0 < (uint64_t)P ? n - p0 : (uint64_t)P);
+    for (uint32_t t = threadIdx.x; t < ns; t += blockDim.x) hist[t] = 0u;
+    if (threadIdx.x == 0) s_ne = 0u;
+    __syncthreads();
+    // the k bit indices of every pattern at least W long (shorter ones set
+    // nothing, ffbf.cpp:271-275), in any order
+    for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) {
+        const uint64_t i = p0 + t;
+        if (off[i + 1] - off[i] < W) continue;
+        const uint2 h = pat_hash[i];
+        const uint32_t at = atomicAdd(&s_ne, k);
+#pragma nounroll
+        for (int j = 0; j < (SPLIT ? kSplitMaxK : 5); j++) {
+            if (j >= (int)k) break;
+            const uint32_t x = SPLIT ? split_place(j, split_fwd_fn(j, h.x, h.y), m) : (fwd_idx(j, h.x, h.y) & mask);
+            raw[at + (uint32_t)j] = x;
+            atomicAdd(&hist[x >> kBinSliceLog2], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t ne = s_ne;
+    // exclusive prefix of the slice counts, in place: each thread sums a run
+    // of consecutive slices, a block scan of the run sums gives its start
+    const uint32_t per = (ns + blockDim.x - 1u) / blockDim.x;
+    const uint32_t lo = threadIdx.x * per < ns ? threadIdx.x * per : ns;
+    const uint32_t hi = lo + per < ns ? lo + per : ns;
+    uint32_t run = 0;
+    for (uint32_t t = lo; t < hi; t++) run += hist[t];
+    const uint32_t incl = wave_incl_sum(run);
+    if (lane_id() == 63) s_wave[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t start = incl - run;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) start += s_wave[w];
+    uint32_t *bo = offs + (uint64_t)blockIdx.x * (ns + 1u);
+    for (uint32_t t = lo; t < hi; t++) {
+        const uint32_t c = hist[t];
+        hist[t] = start;
+        bo[t] = start;
+        start += c;
+    }
+    if (threadIdx.x == 0) bo[ns] = ne;
+    __syncthreads();
+    // counting sort by slice, then one contiguous write
+    for (uint32_t e = threadIdx.x; e < ne; e += blockDim.x) {
+        const uint32_t x = raw[e];
+        sorted[atomicAdd(&hist[x >> kBinSliceLog2], 1u)] = x;
+    }
+    __syncthreads();
+    uint32_t *dst = staging + (uint64_t)blockIdx.x * kBinEntries;
+    for (uint32_t e = threadIdx.x; e < ne; e += blockDim.x) dst[e] = sorted[e];
+}
+
+// One workgroup per 2^20-bit slice of F: the slice's bit indices from every
+// block of k_build_bin set in LDS, then the slice written out whole.
+__global__ __launch_bounds__(1024) void k_build_slices(const uint32_t *staging, const uint32_t *offs, uint32_t nb,
+                                                       uint32_t ns, uint32_t *F, uint64_t f_words)
+{
+    extern __shared__ uint32_t s_slice[];  // 2^15 words
+    constexpr uint32_t kWords = 1u << (kBinSliceLog2 - 5);
+    const uint32_t sl = blockIdx.x;
+    const uint64_t w0 = (uint64_t)sl * kWords;
+    const uint32_t nw = (uint32_t)(f_words - w0 < (uint64_t)kWords ? f_words - w0 : (uint64_t)kWords);
+    for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) s_slice[i] = 0u;
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        const uint32_t *bo = offs + (uint64_t)b * (ns + 1u);
+        const uint32_t a = bo[sl], e = bo[sl + 1u];
+        const uint32_t *src = staging + (uint64_t)b * kBinEntries;
+        uint32_t x = a;
+        for (; x + 8u <= e; x += 8u) {
+            uint32_t v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = src[x + (uint32_t)i];
+#pragma unroll
+            for (int i = 0; i < 8; i++) atomicOr(&s_slice[(v[i] >> 5) & (kWords - 1u)], 1u << (v[i] & 31u));
+        }
+        for (; x < e; x++) {
+            const uint32_t v = src[x];
+            atomicOr(&s_slice[(v >> 5) & (kWords - 1u)], 1u << (v & 31u));
+        }
+    }
+    __syncthreads();
+    uint32_t *dst = F + w0;
+    if ((nw & 3u) == 0u && ((uintptr_t)dst & 15u) == 0u) {
+        for (uint32_t i = threadIdx.x; i < nw / 4u; i += blockDim.x)
+            ((uint4 *)dst)[i] = make_uint4(s_slice[4u * i], s_slice[4u * i + 1u], s_slice[4u * i + 2u], s_slice[4u * i + 3u]);
+    } else {
+        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = s_slice[i];
     }
 }
 
@@ -411,6 +524,38 @@ hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_
     if (blocks > 8192u) blocks = 8192u;
     hipLaunchKernelGGL(k_build_split, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, (const uint2 *)sbox64,
                        W, k, m, F, pat_hash);
+    return hipGetLastError();
+}
+
+uint64_t build_bin_blocks(uint64_t n, uint32_t k) { return (n + build_bin_patterns(k) - 1u) / build_bin_patterns(k); }
+
+hipError_t launch_build_binned(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W, uint32_t k,
+                               uint32_t mask, bool split, SplitMap m, uint32_t *F, uint64_t f_words, uint32_t *staging,
+                               uint32_t *offs, hipStream_t s)
+{
+    const uint64_t ns = (f_words + (1u << (kBinSliceLog2 - 5)) - 1u) >> (kBinSliceLog2 - 5);
+    const uint64_t nb = build_bin_blocks(n, k);
+    if (ns > 4096u || nb > 0xFFFFFFFFu) return hipErrorInvalidValue;
+    const uint32_t P = build_bin_patterns(k);
+    if (nb) {
+        const size_t lds = (2u * kBinEntries + (size_t)ns) * 4u;
+        hipError_t e = hipFuncSetAttribute(split ? (const void *)k_build_bin<true> : (const void *)k_build_bin<false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        if (split)
+            hipLaunchKernelGGL(k_build_bin<true>, dim3((uint32_t)nb), dim3(256), lds, s, off, pat_hash, n, P, W, k, mask,
+                               m, (uint32_t)ns, staging, offs);
+        else
+            hipLaunchKernelGGL(k_build_bin<false>, dim3((uint32_t)nb), dim3(256), lds, s, off, pat_hash, n, P, W, k,
+                               mask, m, (uint32_t)ns, staging, offs);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const size_t lds = (size_t)(1u << (kBinSliceLog2 - 5)) * 4u;
+    hipError_t e = hipFuncSetAttribute((const void *)k_build_slices, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_build_slices, dim3((uint32_t)ns), dim3(1024), lds, s, staging, offs, (uint32_t)nb,
+                       (uint32_t)ns, F, f_words);
     return hipGetLastError();
 }
 
