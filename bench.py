@@ -262,7 +262,10 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
-def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20, reps=5):
+UPLOADS = {"default": 0, "copy": 1, "kernel": 2}
+
+
+def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20, reps=5, upload=0):
     """Continuous-batching server vs the plain per-request path (one
     ffbf_match_text per request, first come first served) on the same
     requests.  Saturation throughput of each (everything queued at once);
@@ -298,7 +301,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     plain_s = time.perf_counter() - t1
     plain_rate = n_requests / plain_s
     t1 = time.perf_counter()
-    srv = fa.Server(ms, hbm_budget_bytes=budget, slots=slots, max_batch_bytes=max_batch_bytes)
+    srv = fa.Server(ms, hbm_budget_bytes=budget, slots=slots, max_batch_bytes=max_batch_bytes, upload=upload)
     create_s = time.perf_counter() - t1
     # the first pass over the new server, then one more before measuring
     t1 = time.perf_counter()
@@ -331,6 +334,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         arrivals = arrivals_at(lam)
         lat = [None] * n_requests
         rids = [None] * n_requests
+        lag = [0.0]
         ready = [threading.Event() for _ in range(n_requests)]
 
         def submitter():
@@ -339,6 +343,8 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
                 dt = arrivals[i] - (time.perf_counter() - start)
                 if dt > 0:
                     time.sleep(dt)
+                elif -dt > lag[0]:
+                    lag[0] = -dt  # submitted late: the arrivals behind it come as a burst
                 rids[i] = srv.submit(f, t)
                 ready[i].set()
 
@@ -349,6 +355,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
             ready[i].wait()
             _, lat[i] = srv.wait(rids[i])
         th.join()
+        drive_server.lag_ms = lag[0] * 1e3
         return lat, time.perf_counter() - t1
 
     def drive_plain(lam):
@@ -386,11 +393,13 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         (p50/p99 over reps x n_requests samples, so that one host stall does
         not make the p99), the mean pass time and each pass's own p99."""
         lat, el, p99s = [], 0.0, []
+        passes.lags = []
         for _ in range(reps):
             l1, e1 = drive(lam)
             lat += l1
             el += e1
             p99s.append(round(percentile(l1, 99), 3))
+            passes.lags.append(round(getattr(drive, "lag_ms", 0.0), 3))
         return lat, el / reps, p99s
 
     # both paths at the same arrival rates, below the plain path's saturation
@@ -398,11 +407,12 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     for ld in (0.5, 0.8):
         lam = ld * plain_rate
         s_lat, s_el, s_p99s = passes(drive_server, lam)
+        s_lags = passes.lags
         p_lat, p_el, p_p99s = passes(drive_plain, lam)
         row = {"load_vs_plain_saturation": ld, "load_vs_server_saturation": round(lam / srv_rate, 4),
                "rate_per_s": round(lam, 1), "server_p50_ms": pct(s_lat)[0], "server_p99_ms": pct(s_lat)[1],
                "plain_p50_ms": pct(p_lat)[0], "plain_p99_ms": pct(p_lat)[1],
-               "server_p99_ms_per_pass": s_p99s, "plain_p99_ms_per_pass": p_p99s,
+               "server_p99_ms_per_pass": s_p99s, "plain_p99_ms_per_pass": p_p99s, "server_submit_lag_ms_per_pass": s_lags,
                "server_gbs": round(total / s_el / 1e9, 3), "plain_gbs": round(total / p_el / 1e9, 3)}
         same_rate.append(row)
         log(f"serve: same arrivals at {lam:.0f} req/s ({ld} of plain saturation): server p99 {row['server_p99_ms']} ms, "
@@ -416,7 +426,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         if ld == load:
             lat_at_load, poisson_s = s_lat, s_el
         row = {"load": ld, "rate_per_s": round(lam, 1), "p50_ms": pct(s_lat)[0], "p99_ms": pct(s_lat)[1],
-               "p99_ms_per_pass": s_p99s,
+               "p99_ms_per_pass": s_p99s, "submit_lag_ms_per_pass": passes.lags,
                "gbs": round(total / s_el / 1e9, 3), "load_vs_plain_saturation": round(lam / plain_rate, 3),
                "plain": "unstable: arrivals above its saturation rate" if lam >= plain_rate else "stable"}
         sweep.append(row)
@@ -450,6 +460,8 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         "same_arrivals": same_rate,
         "load_sweep": sweep,
         "batches": int(stats["batches"]),
+        "upload": {0: "default", 1: "copy", 2: "kernel"}.get(upload, str(upload)),
+        "kernel_upload_bytes": int(stats["kernel_upload_bytes"]),
         "hbm_budget_bytes": budget,
         "slots": slots,
         "max_batch_bytes": max_batch_bytes,
@@ -1017,6 +1029,8 @@ def main() -> int:
     ap.add_argument("--probe-sweep", action="store_true",
                     help="only run the access-kind microbenchmark (random loads of four cache policies and "
                          "device-scope atomicOr into 1 MiB..256 MiB; streaming passes over 16 MiB..4 GiB) and exit")
+    ap.add_argument("--serve-upload", choices=["default", "copy", "kernel"], default="default",
+                    help="how the serving leg's request text reaches the device (server upload method)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the scan's HBM traffic")
     ap.add_argument("--traffic-child", type=int, default=0, metavar="BYTES", help=argparse.SUPPRESS)
@@ -1107,7 +1121,7 @@ def main() -> int:
         return 0
 
     if args.workload == "serve-8sets":
-        r = run_serving(fa, n_requests=max(64, args.steps * 100))
+        r = run_serving(fa, n_requests=max(64, args.steps * 100), upload=UPLOADS[args.serve_upload])
         print(json.dumps({"metric": METRIC, "value": r["saturation_gbs"], "unit": "GB/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "dtype": "u32",
@@ -1140,7 +1154,7 @@ def main() -> int:
         out["batched_multi_filter"] = run_batched(fa, "batch-2048texts-64sets", steps=5, per_text_reps=1,
                                                   cpu=not args.no_cpu)
         log("serving leg (serve-8sets) ...")
-        out["serving"] = run_serving(fa, n_requests=600)
+        out["serving"] = run_serving(fa, n_requests=600, upload=UPLOADS[args.serve_upload])
         log("ingest leg (host memory / file) ...")
         out["ingest"] = run_ingest(fa, reps=2)
         log("split filter leg (split-1m-2g-lines) ...")

@@ -692,12 +692,18 @@ class ServerConfig(ctypes.Structure):
         ("max_batch_bytes", ctypes.c_uint64),
         ("slots", ctypes.c_uint32),
         ("max_batch_requests", ctypes.c_uint32),
+        ("upload", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
+
+
+UPLOAD_DEFAULT, UPLOAD_COPY, UPLOAD_KERNEL = 0, 1, 2
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("submitted", "completed", "rejected", "batches", "bytes", "pending", "in_flight_bytes")]
+                ("submitted", "completed", "rejected", "batches", "bytes", "pending", "in_flight_bytes",
+                 "kernel_upload_bytes")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -709,9 +715,9 @@ class Server:
     matches and the submit-to-completion latency."""
 
     def __init__(self, matchers: Sequence[Matcher], hbm_budget_bytes: int = 0, slots: int = 0,
-                 max_batch_requests: int = 0, max_batch_bytes: int = 0):
+                 max_batch_requests: int = 0, max_batch_bytes: int = 0, upload: int = UPLOAD_DEFAULT):
         arr = (_c.c_void_p * len(matchers))(*[m._h.value for m in matchers])
-        cfg = ServerConfig(hbm_budget_bytes, max_batch_bytes, slots, max_batch_requests)
+        cfg = ServerConfig(hbm_budget_bytes, max_batch_bytes, slots, max_batch_requests, upload, 0)
         h = _c.c_void_p(0)
         _check(_lib.ffbf_server_create(arr, len(matchers), _c.byref(cfg), _c.byref(h)))
         self._h = h

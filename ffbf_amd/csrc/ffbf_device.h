@@ -211,9 +211,32 @@ struct ZeroSpan {
     uint32_t words;
 };
 
+// Pieces of host text, each in one device-mapped host allocation (the GPU
+// reads it at the host address), copied back to back into a device buffer
+// by one launch.  The work is cut into segments of 63 16-byte destination
+// chunks (one wave each): piece p has segments [seg_start[p],
+// seg_start[p + 1]).
+constexpr uint32_t kUploadMax = 64;  // pieces per launch (1.8 KiB of kernel arguments)
+struct UploadArgs {
+    uint8_t *dst;
+    uint32_t n;           // pieces in this launch
+    uint32_t total_segs;  // seg_start[n]
+    uint64_t src[kUploadMax];
+    uint64_t dst_off[kUploadMax];
+    uint64_t len[kUploadMax];
+    uint32_t seg_start[kUploadMax + 1];
+};
+// Segments of a piece of n > 0 bytes at destination offset dst_off.
+inline uint32_t upload_segs(uint64_t dst_off, uint64_t n)
+{
+    const uint64_t chunks = ((dst_off + n - 1u) >> 4) - (dst_off >> 4) + 1u;
+    return (uint32_t)((chunks + 62u) / 63u);
+}
+
 // Launchers (hipError_t of the launch).
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
+hipError_t launch_upload(const UploadArgs &a, hipStream_t s);
 hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
                         uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, hipStream_t s);
 // The split method's build: 64-bit prefix hashes from a 256-entry (lo, hi)

@@ -484,6 +484,83 @@ __global__ __launch_bounds__(256) void k_query_lines(QueryArgs a)
     }
 }
 
+// Host text into device memory (see UploadArgs).  Lane L of a segment's
+// wave owns destination chunk c = first chunk + 63 * segment + L and loads
+// the 16-aligned source chunk that holds the source of c's first byte; the
+// chunk after it is lane L + 1's, so lanes 0..62 each assemble one
+// destination chunk from two loaded ones (source and destination differ by
+// the same shift r within a piece) and lane 63 only loads.  Source reads
+// are buffer loads over the piece's 16-aligned extent: a chunk with no byte
+// of the piece is out of the buffer's range and reads as zeros without an
+// access, so no read leaves the pages the piece lies in.  They are
+// system-scope (sc0 sc1) loads, so the host's latest bytes are read, never
+// a cached copy.  Chunks cut by a piece's ends are written byte by byte,
+// which leaves the neighbouring piece's bytes of a shared chunk alone.
+__global__ __launch_bounds__(256) void k_upload(const UploadArgs a)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const uint32_t lane = threadIdx.x & 63u;
+    // wave-uniform: the segment search and every per-piece value stay scalar
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t g = wave; g < a.total_segs; g += n_waves) {
+        uint32_t lo = 0, hi = a.n;  // the piece: seg_start[lo] <= g < seg_start[lo + 1]
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.seg_start[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t sp = a.src[lo], dp = a.dst_off[lo], n = a.len[lo];
+        const uint64_t base = sp & ~15ull;
+        const uint32_t extent = (uint32_t)(((sp + n + 15u) & ~15ull) - base);
+        const uint64_t delta = sp - dp;  // source address of destination offset x: x + delta
+        const uint32_t r = (uint32_t)(delta & 15u);
+        const uint64_t c = (dp >> 4) + 63ull * (g - a.seg_start[lo]) + lane;
+        // 16-aligned source chunk of c's first byte, relative to base (a
+        // chunk before base wraps to a large offset: out of range as well)
+        const uint32_t off = (uint32_t)(16ull * c + (delta - r) - base);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)extent, 0x00020000);
+        const u32x4 cur = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 17);  // sc0 sc1
+        u32x4 nxt;
+        nxt.x = __shfl_down(cur.x, 1);
+        nxt.y = __shfl_down(cur.y, 1);
+        nxt.z = __shfl_down(cur.z, 1);
+        nxt.w = __shfl_down(cur.w, 1);
+        const uint64_t d0 = 16ull * c;
+        const uint64_t b_lo = d0 > dp ? d0 : dp, b_hi = d0 + 16u < dp + n ? d0 + 16u : dp + n;
+        if (lane == 63u || b_lo >= b_hi) continue;
+        // bytes r .. r + 15 of cur:nxt (r is the same for the whole wave)
+        const uint32_t w0 = cur.x, w1 = cur.y, w2 = cur.z, w3 = cur.w, w4 = nxt.x, w5 = nxt.y, w6 = nxt.z, w7 = nxt.w;
+        const uint32_t sh = r & 3u;
+        u32x4 o;
+        switch (r >> 2) {
+        case 0:
+            o = u32x4{__builtin_amdgcn_alignbyte(w1, w0, sh), __builtin_amdgcn_alignbyte(w2, w1, sh),
+                      __builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh)};
+            break;
+        case 1:
+            o = u32x4{__builtin_amdgcn_alignbyte(w2, w1, sh), __builtin_amdgcn_alignbyte(w3, w2, sh),
+                      __builtin_amdgcn_alignbyte(w4, w3, sh), __builtin_amdgcn_alignbyte(w5, w4, sh)};
+            break;
+        case 2:
+            o = u32x4{__builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh),
+                      __builtin_amdgcn_alignbyte(w5, w4, sh), __builtin_amdgcn_alignbyte(w6, w5, sh)};
+            break;
+        default:
+            o = u32x4{__builtin_amdgcn_alignbyte(w4, w3, sh), __builtin_amdgcn_alignbyte(w5, w4, sh),
+                      __builtin_amdgcn_alignbyte(w6, w5, sh), __builtin_amdgcn_alignbyte(w7, w6, sh)};
+            break;
+        }
+        if (b_lo == d0 && b_hi == d0 + 16u) {
+            *(u32x4 *)(a.dst + d0) = o;
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < 16u; j++)
+                if (d0 + j >= b_lo && d0 + j < b_hi) a.dst[d0 + j] = (uint8_t)(o[j >> 2] >> (8u * (j & 3u)));
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s)
@@ -503,6 +580,18 @@ hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s
     uint64_t blocks = (n16 + 255u) / 256u;
     if (blocks > 8192u) blocks = 8192u;
     hipLaunchKernelGGL(k_copy, dim3((uint32_t)blocks), dim3(256), 0, s, (uint4 *)dst, (const uint4 *)src, n16);
+    return hipGetLastError();
+}
+
+hipError_t launch_upload(const UploadArgs &a, hipStream_t s)
+{
+    if (a.n == 0 || a.n > kUploadMax || a.seg_start[a.n] != a.total_segs) return hipErrorInvalidValue;
+    for (uint32_t p = 0; p < a.n; p++)  // one 32-bit buffer range per piece
+        if (!a.len[p] || a.len[p] > (1ull << 31)) return hipErrorInvalidValue;
+    // 32-64 workgroups read the link at its full rate (57 GB/s, the same as
+    // the DMA engine); fewer for small batches
+    const uint32_t blocks = std::min<uint32_t>(64u, (a.total_segs + 3u) / 4u);
+    hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ void Scheduler::shutdown()
 }
 
 int Scheduler::submit(uint32_t filter, const uint8_t *text, uint64_t n, uint64_t overlap, uint64_t *id,
-                      std::string &err)
+                      std::string &err, bool device_readable)
 {
     const uint64_t cap = cfg_.slot_bytes;
     if (n > cap && (overlap == 0 || 2 * overlap > cap)) {
@@ -64,6 +64,7 @@ int Scheduler::submit(uint32_t filter, const uint8_t *text, uint64_t n, uint64_t
         const bool last = start + p.n >= n;
         p.own = last ? p.n : stride;
         p.footprint = piece_footprint(p.n);
+        p.device_readable = device_readable;
         r->pieces.push_back(std::move(p));
         if (last) break;
         start += stride;

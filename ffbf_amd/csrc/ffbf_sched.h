@@ -39,6 +39,7 @@ struct SchedPiece {
     uint64_t start = 0, n = 0;
     uint64_t own = 0;        // window starts this piece reports: local offsets below own
     uint64_t footprint = 0;  // device bytes held while in flight
+    bool device_readable = false;  // the text lies in device-mapped host memory
     std::vector<ffbf_match> out;  // filled by the executor: matches at local offsets
 };
 
@@ -86,7 +87,10 @@ class Scheduler {
     void shutdown();
 
     // overlap: bytes consecutive pieces share (at most slot_bytes / 2).
-    int submit(uint32_t filter, const uint8_t *text, uint64_t n, uint64_t overlap, uint64_t *id, std::string &err);
+    // device_readable: the device may read the text at its host address
+    // (passed on to every piece).
+    int submit(uint32_t filter, const uint8_t *text, uint64_t n, uint64_t overlap, uint64_t *id, std::string &err,
+               bool device_readable = false);
     // FFBF_OK: the request will complete with FFBF_ECANCELED; FFBF_ESTATE: it
     // had completed already (its result stands); FFBF_EINVAL: unknown id.
     int cancel(uint64_t id, std::string &err);

@@ -16,6 +16,7 @@
 // reported by the piece owning its offset.  A request's matches are those of
 // ffbf_run_device on its text alone: verification is per piece and piece
 // borders never cut a match.
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -35,6 +36,7 @@ struct Slot {
     std::vector<uint64_t> offsets;
     std::vector<uint32_t> route;
     std::vector<ffbf_batch_match> bm;
+    UploadArgs upload{};  // the copy kernel's arguments for the current launch
 };
 
 // Device-to-device copy of a built context: same parameters, patterns,
@@ -69,10 +71,36 @@ int clone_ctx(const ffbf_ctx *src, ffbf_ctx **out, uint64_t *max_len)
     return ffbf_synchronize(c);
 }
 
+// Whether the device may read text[0, n) at its host address: the bytes lie
+// wholly in one host allocation mapped to the device at the same address
+// (hipHostMalloc memory).  Anything else (pageable memory, device memory,
+// a range running past its allocation) answers false.
+bool device_readable(int device, const uint8_t *text, uint64_t n)
+{
+    if (!text || !n) return false;
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, text) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (at.type != hipMemoryTypeHost || at.devicePointer != (const void *)text || at.device != device) return false;
+    void *base = nullptr;
+    size_t size = 0;
+    if (hipPointerGetAttribute(&base, HIP_POINTER_ATTRIBUTE_RANGE_START_ADDR, (hipDeviceptr_t)text) != hipSuccess ||
+        hipPointerGetAttribute(&size, HIP_POINTER_ATTRIBUTE_RANGE_SIZE, (hipDeviceptr_t)text) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uintptr_t b = (uintptr_t)base, t = (uintptr_t)text;
+    return b && t >= b && n <= size && t - b <= size - n;
+}
+
 class GpuExecutor : public SchedExecutor {
   public:
     std::vector<Slot> slots;
     int device = 0;
+    bool upload_kernel = false;
+    std::atomic<uint64_t> kernel_bytes{0};
 
     ~GpuExecutor() override
     {
@@ -92,16 +120,43 @@ class GpuExecutor : public SchedExecutor {
         sl.offsets.assign(work.size() + 1, 0);
         sl.route.resize(work.size());
         const hipStream_t stream = batch_stream(sl.batch);
+        // pieces in device-mapped host memory go by the copy kernel, up to
+        // kUploadMax per launch; the rest by the DMA engine
+        UploadArgs &ua = sl.upload;
+        ua.dst = sl.d_text;
+        ua.n = 0;
+        ua.total_segs = 0;
+        uint64_t by_kernel = 0;
+        auto flush = [&]() -> bool {
+            if (!ua.n) return true;
+            ua.seg_start[ua.n] = ua.total_segs;
+            const hipError_t e = launch_upload(ua, stream);
+            ua.n = 0;
+            ua.total_segs = 0;
+            if (e == hipSuccess) return true;
+            err = std::string("text upload kernel: ") + hipGetErrorString(e);
+            return false;
+        };
         for (size_t i = 0; i < work.size(); i++) {
-            if (work[i]->n &&
-                hipMemcpyAsync(sl.d_text + sl.offsets[i], work[i]->text, work[i]->n, hipMemcpyHostToDevice, stream) !=
-                    hipSuccess) {
+            const uint64_t n = work[i]->n, at = sl.offsets[i];
+            if (n && upload_kernel && work[i]->device_readable) {
+                ua.src[ua.n] = (uint64_t)(uintptr_t)work[i]->text;
+                ua.dst_off[ua.n] = at;
+                ua.len[ua.n] = n;
+                ua.seg_start[ua.n] = ua.total_segs;
+                ua.total_segs += upload_segs(at, n);
+                by_kernel += n;
+                if (++ua.n == kUploadMax && !flush()) return FFBF_EDEVICE;
+            } else if (n && hipMemcpyAsync(sl.d_text + at, work[i]->text, n, hipMemcpyHostToDevice, stream) !=
+                                hipSuccess) {
                 err = std::string("text upload: ") + hipGetErrorString(hipGetLastError());
                 return FFBF_EDEVICE;
             }
-            sl.offsets[i + 1] = sl.offsets[i] + work[i]->n;
+            sl.offsets[i + 1] = at + n;
             sl.route[i] = work[i]->filter;
         }
+        if (!flush()) return FFBF_EDEVICE;
+        kernel_bytes += by_kernel;
         uint64_t nm = 0;
         int st = ffbf_batch_set_texts(sl.batch, sl.d_text, sl.offsets.data(), sl.route.data(), (uint32_t)work.size());
         if (st == FFBF_OK) st = ffbf_batch_run(sl.batch, &nm);
@@ -158,9 +213,14 @@ int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_
     if (!s->cfg.max_batch_requests) s->cfg.max_batch_requests = 512;
     if (!s->cfg.max_batch_bytes)
         s->cfg.max_batch_bytes = std::min<uint64_t>(s->cfg.hbm_budget_bytes / s->cfg.slots, 128ull << 20);
+    if (s->cfg.upload > FFBF_UPLOAD_KERNEL) return fail(FFBF_EINVAL, "unknown upload method");
+    // the copy kernel addresses a piece with one 32-bit buffer range
+    if (s->cfg.upload == FFBF_UPLOAD_KERNEL && s->cfg.max_batch_bytes > (1ull << 31))
+        return fail(FFBF_EINVAL, "the upload kernel needs max_batch_bytes of at most 2 GiB");
     s->exec.reset(new (std::nothrow) GpuExecutor());
     if (!s->exec) return fail(FFBF_ENOMEM, "server allocation");
     s->exec->device = filters[0]->device;
+    s->exec->upload_kernel = s->cfg.upload == FFBF_UPLOAD_KERNEL;
     s->exec->slots.resize(s->cfg.slots);
     s->overlap.assign(n_filters, 0);
     for (uint32_t k = 0; k < s->cfg.slots; k++) {
@@ -223,7 +283,8 @@ int ffbf_server_submit(ffbf_server *s, uint32_t filter, const uint8_t *text, uin
     if (filter >= s->overlap.size()) return fail(FFBF_EINVAL, "filter id out of range");
     if (!text && n) return fail(FFBF_EINVAL, "null text");
     std::string err;
-    const int st = s->sched->submit(filter, text, n, s->overlap[filter], request_id, err);
+    const bool readable = s->exec->upload_kernel && device_readable(s->exec->device, text, n);
+    const int st = s->sched->submit(filter, text, n, s->overlap[filter], request_id, err, readable);
     return st == FFBF_OK ? st : fail(st, err);
 }
 
@@ -272,6 +333,7 @@ int ffbf_server_get_stats(ffbf_server *s, ffbf_server_stats *out)
 {
     if (!s || !out) return fail(FFBF_EINVAL, "null argument");
     s->sched->stats(out);
+    out->kernel_upload_bytes = s->exec->kernel_bytes.load();
     return FFBF_OK;
 }
 

@@ -355,10 +355,20 @@ typedef struct ffbf_server_config {
     uint64_t max_batch_bytes;    /* text bytes per batch = per slot buffer (0 = min(budget / slots, 128 MiB)) */
     uint32_t slots;              /* batches in flight at once (0 = 4) */
     uint32_t max_batch_requests; /* requests per batch (0 = 512) */
+    uint32_t upload;             /* how request text reaches the device: FFBF_UPLOAD_* */
+    uint32_t reserved;           /* 0 */
 } ffbf_server_config;
+
+/* FFBF_UPLOAD_COPY: hipMemcpyAsync per piece (the DMA engine).
+ * FFBF_UPLOAD_KERNEL: a copy kernel on the slot's stream reads text that
+ * lies wholly in one device-mapped host allocation (hipHostMalloc,
+ * ffbf_host_alloc) over the link; other text still goes by
+ * hipMemcpyAsync.  FFBF_UPLOAD_DEFAULT (0) is FFBF_UPLOAD_COPY. */
+enum { FFBF_UPLOAD_DEFAULT = 0, FFBF_UPLOAD_COPY = 1, FFBF_UPLOAD_KERNEL = 2 };
 
 typedef struct ffbf_server_stats {
     uint64_t submitted, completed, rejected, batches, bytes, pending, in_flight_bytes;
+    uint64_t kernel_upload_bytes; /* request bytes the copy kernel uploaded */
 } ffbf_server_stats;
 
 int ffbf_server_create(ffbf_ctx *const *filters, uint32_t n_filters, const ffbf_server_config *config,

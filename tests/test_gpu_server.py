@@ -124,6 +124,70 @@ def test_server_pinned_requests():
         f[0].close()
 
 
+def dense_text(b, off, seed, size):
+    """Patterns of the set back to back: every byte lies in a planted
+    occurrence, so one wrong byte anywhere changes the match list."""
+    rng = np.random.default_rng(seed)
+    lens = np.diff(off)
+    parts, total = [], 0
+    while total < size:
+        j = int(rng.integers(0, lens.size))
+        parts.append(b[int(off[j]):int(off[j + 1])])
+        total += int(lens[j])
+    return np.concatenate(parts)[:size] if parts else np.zeros(0, dtype=np.uint8)
+
+
+def test_server_upload_kernel():
+    """Text in pinned host memory uploaded by the server's copy kernel: every
+    source alignment mod 16, lengths around the kernel's 16-byte chunks and
+    63-chunk segments, a request cut into pieces, batches packing pieces at
+    odd device offsets.  Matches equal each request scanned alone; pageable
+    text still goes by the DMA engine."""
+    say("server upload kernel")
+    sbox = fa.sbox_from_seed(94)
+    filters = make_filters(sbox)[:2]
+    sizes = [1, 2, 15, 16, 17, 31, 33, 1007, 1008, 1009, 1024, 4099, 65543, 300_001, 3 << 20]
+    sizes += [int(x) for x in np.random.default_rng(5).integers(1, 200_000, size=33)]
+    pin = fa.PinnedBuffer(sum(sizes) + 32 * len(sizes) + 64)
+    reqs, at = [], 0
+    for i, n in enumerate(sizes):
+        at = (at + 15) // 16 * 16 + i % 16  # source addresses at every alignment mod 16
+        f = i % 2
+        _, b, off, _, _ = filters[f]
+        t = dense_text(b, off, 40 + i, n)
+        pin.array[at:at + n] = t
+        reqs.append((f, at, n))
+        at += n
+    expect = [alone(filters, f, pin.array[a:a + n].copy()) for f, a, n in reqs]
+    assert sum(e.size for e in expect) > 10_000
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=2, max_batch_bytes=2 << 20,
+                   upload=fa.UPLOAD_KERNEL) as srv:
+        for rep in range(2):
+            ids = [srv.submit(f, pin.array[a:a + n]) for f, a, n in reqs]
+            for i, rid in enumerate(ids):
+                got, _ = srv.wait(rid)
+                assert np.array_equal(got["pattern_id"], expect[i]["pattern_id"]), (rep, i, reqs[i])
+                assert np.array_equal(got["offset"], expect[i]["offset"]), (rep, i, reqs[i])
+        up = srv.stats()["kernel_upload_bytes"]
+        assert up >= 2 * sum(sizes)  # every pinned byte (pieces of the long one overlap)
+        # pageable copies of the same requests: the DMA engine, same matches
+        ids = [srv.submit(f, pin.array[a:a + n].copy()) for f, a, n in reqs[:12]]
+        for i, rid in enumerate(ids):
+            got, _ = srv.wait(rid)
+            assert np.array_equal(got["offset"], expect[i]["offset"]), i
+        assert srv.stats()["kernel_upload_bytes"] == up
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=1, max_batch_bytes=2 << 20) as srv:
+        got, _ = srv.wait(srv.submit(reqs[13][0], pin.array[reqs[13][1]:reqs[13][1] + reqs[13][2]]))
+        assert np.array_equal(got["offset"], expect[13]["offset"])
+        assert srv.stats()["kernel_upload_bytes"] == 0  # the default copies
+    with pytest.raises(fa.FFBFError) as e:
+        fa.Server([filters[0][0]], upload=7)
+    assert e.value.status == fa.EINVAL
+    pin.free()
+    for f in filters:
+        f[0].close()
+
+
 def test_server_admission_limits():
     say("server admission")
     sbox = fa.sbox_from_seed(92)
