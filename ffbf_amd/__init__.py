@@ -42,6 +42,7 @@ OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY, ECANCELED = -1, -2, -3, -4, -5, -6
 FILTER_FORWARD, FILTER_REVERSE = 0, 1
 MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS, MODE_GFOLD, MODE_HYBRID = -1, 0, 1, 2, 3, 4
+BUILD_AUTO, BUILD_ATOMIC, BUILD_BINNED = 0, 1, 2
 ALPHA_ASCII, ALPHA_DNA_GC40 = 0, 1
 
 
@@ -150,6 +151,7 @@ ABI = {
     "ffbf_filter_load": (_i32, [_P, _P, _u64]),
     "ffbf_set_scan_mode": (_i32, [_P, _i32, _u32]),
     "ffbf_set_scan_tuning": (_i32, [_P, _u32, _u32]),
+    "ffbf_set_build_method": (_i32, [_P, _i32]),
     "ffbf_gather_bench": (_i32, [_i32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_probe_bench": (_i32, [_i32, _u32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_get_timings": (_i32, [_P, _c.POINTER(Timings)]),
@@ -458,6 +460,10 @@ class Matcher:
 
     def set_scan_tuning(self, probes: int = 0, block: int = 0) -> None:
         _check(_lib.ffbf_set_scan_tuning(self._h, probes, block))
+
+    def set_build_method(self, method: int) -> None:
+        """BUILD_AUTO, BUILD_ATOMIC or BUILD_BINNED, from the next build()."""
+        _check(_lib.ffbf_set_build_method(self._h, method))
 
     # -- scan / pass / verify
     def scan(self, text) -> int:

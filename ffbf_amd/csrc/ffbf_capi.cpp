@@ -857,10 +857,16 @@ int ffbf_build(ffbf_ctx *c)
     if (!c->patterns) return fail(FFBF_ESTATE, "build before patterns are loaded");
     RET_IF(record(c, EV_B0));
     const uint64_t slices = (c->f_words + (1u << (ffbf::kBinSliceLog2 - 5)) - 1u) >> (ffbf::kBinSliceLog2 - 5);
-    if (slices >= 2) {
-        // F of several 2^20-bit slices: bits sorted by slice and set in LDS,
-        // each slice written whole (no atomics, no clear), from the prefix
-        // hash pairs ffbf_set_patterns computed
+    // device-scope atomics run at ~27 G/s whatever F's size: below 2^20 bit
+    // sets (~39 us of them) the clear and the atomics beat the binned
+    // build's two passes
+    const bool binned = c->build_method == FFBF_BUILD_BINNED ||
+                        (c->build_method == FFBF_BUILD_AUTO && slices >= 2 &&
+                         c->n_long * c->params.k >= (1ull << 20));
+    if (binned) {
+        // bits sorted by 2^20-bit slice and set in LDS, each slice written
+        // whole (no atomics, no clear), from the prefix hash pairs
+        // ffbf_set_patterns computed
         const uint64_t nb = ffbf::build_bin_blocks(c->n_pat, c->params.k);
         RET_IF(c->bin_staging.ensure(std::max<uint64_t>(nb, 1) * ffbf::kBinEntries, "build staging"));
         RET_IF(c->bin_offs.ensure(std::max<uint64_t>(nb, 1) * (slices + 1u), "build slice offsets"));
@@ -881,6 +887,14 @@ int ffbf_build(ffbf_ctx *c)
     RET_IF(choose_mode(c));
     c->built = true;
     c->scanned = c->passed = c->verified = false;
+    return FFBF_OK;
+}
+
+int ffbf_set_build_method(ffbf_ctx *c, int method)
+{
+    RET_IF(check_ctx(c));
+    if (method < FFBF_BUILD_AUTO || method > FFBF_BUILD_BINNED) return fail(FFBF_EINVAL, "build method must be 0..2");
+    c->build_method = method;
     return FFBF_OK;
 }
 

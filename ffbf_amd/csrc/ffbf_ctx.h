@@ -137,6 +137,7 @@ struct ffbf_ctx {
     int mode_req = -1;
     uint32_t fold_log2_req = 0;
     uint32_t probes_req = 0, block_req = 0;
+    int build_method = FFBF_BUILD_AUTO;
     uint32_t mode = ffbf::MODE_GLOBAL;
     uint32_t pre_words = 0;   // prefilter size in words (F itself, LDS fold or global fold)
     uint32_t gfold_words = 0; // size of the L2-resident fold behind the LDS fold (mode 4)

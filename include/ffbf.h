@@ -278,6 +278,17 @@ int ffbf_set_scan_mode(ffbf_ctx *ctx, int mode, uint32_t fold_log2_bits);
  * ffbf_get_timings reports the effective values. */
 int ffbf_set_scan_tuning(ffbf_ctx *ctx, uint32_t probes, uint32_t block);
 
+/* How ffbf_build sets F's bits (F is the same either way); no counterpart in
+ * the reference, whose grepSetup sets one bit per atomic (greptest.cu:96-123).
+ * FFBF_BUILD_ATOMIC: F cleared, then one device-scope atomicOr per bit.
+ * FFBF_BUILD_BINNED: bit indices counting-sorted by 2^20-bit slice, each
+ * slice set in LDS and written whole (no atomics on F, no clear).
+ * FFBF_BUILD_AUTO (0): binned for filters of two slices or more with at
+ * least 2^20 bit sets (where the atomics would dominate), else atomic.
+ * Takes effect at the next ffbf_build. */
+enum { FFBF_BUILD_AUTO = 0, FFBF_BUILD_ATOMIC = 1, FFBF_BUILD_BINNED = 2 };
+int ffbf_set_build_method(ffbf_ctx *ctx, int method);
+
 int ffbf_get_timings(ffbf_ctx *ctx, ffbf_timings *out);
 int ffbf_synchronize(ffbf_ctx *ctx);
 
