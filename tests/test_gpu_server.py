@@ -176,6 +176,38 @@ def test_server_upload_kernel():
             got, _ = srv.wait(rid)
             assert np.array_equal(got["offset"], expect[i]["offset"]), i
         assert srv.stats()["kernel_upload_bytes"] == up
+    # 300 small pinned requests queued behind eight pageable 2 MiB ones on a
+    # one-slot server: they wait together and go in a few batches of more
+    # than 64 pieces each, so several upload launches per batch
+    small = []
+    tiny = fa.PinnedBuffer(300 * 4200)
+    at = 0
+    for i in range(300):
+        n = 1 + (i * 37) % 4000
+        at = (at + 15) // 16 * 16 + (i * 7) % 16
+        f = i % 2
+        _, b, off, _, _ = filters[f]
+        tiny.array[at:at + n] = dense_text(b, off, 900 + i, n)
+        small.append((f, at, n))
+        at += n
+    want = [alone(filters, f, tiny.array[a:a + n].copy()) for f, a, n in small]
+    blockers = [(i % 2, fa.gen_bytes(fa.ALPHA_ASCII, 950 + i, 2 << 20)) for i in range(8)]
+    with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=1, max_batch_bytes=2 << 20,
+                   upload=fa.UPLOAD_KERNEL) as srv:
+        first = srv.submit_many(blockers)
+        ids = srv.submit_many([(f, tiny.array[a:a + n]) for f, a, n in small])
+        for rid in first:
+            srv.wait(rid)
+        b0 = srv.stats()["batches"]
+        for i, rid in enumerate(ids):
+            got, _ = srv.wait(rid)
+            assert np.array_equal(got["pattern_id"], want[i]["pattern_id"]), (i, small[i])
+            assert np.array_equal(got["offset"], want[i]["offset"]), (i, small[i])
+        tiny_batches = srv.stats()["batches"] - b0
+        say(f"  300 small requests in {tiny_batches} batches")
+        assert tiny_batches <= 4  # so some batch held at least 75 pieces
+        assert srv.stats()["kernel_upload_bytes"] == sum(n for _, _, n in small)
+    tiny.free()
     with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=1, max_batch_bytes=2 << 20) as srv:
         got, _ = srv.wait(srv.submit(reqs[13][0], pin.array[reqs[13][1]:reqs[13][1] + reqs[13][2]]))
         assert np.array_equal(got["offset"], expect[13]["offset"])
