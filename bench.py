@@ -1058,7 +1058,8 @@ def main() -> int:
                     help="only run the random-gather microbenchmark into a 2^L-byte array (two launches) and exit")
     ap.add_argument("--probe-sweep", action="store_true",
                     help="only run the access-kind microbenchmark (random loads of four cache policies and "
-                         "device-scope atomicOr into 1 MiB..256 MiB; streaming passes over 16 MiB..4 GiB) and exit")
+                         "device-scope atomicOr into 1 MiB..256 MiB; streaming passes over 16 MiB..4 GiB; "
+                         "pinned-host uploads by the upload kernel of 1 MiB..1 GiB) and exit")
     ap.add_argument("--serve-upload", choices=["default", "copy", "kernel"], default="default",
                     help="how the serving leg's request text reaches the device (server upload method)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -1121,6 +1122,10 @@ def main() -> int:
         for l in range(24, 33):
             ms, rate = fa.probe_bench(fa.PROBE_STREAM, l, steps=max(4, 1 << (33 - l)))
             print(json.dumps({"kind": fa.PROBE_KIND_NAMES[fa.PROBE_STREAM], "array_bytes": 1 << l,
+                              "ms": round(ms, 4), "bytes_per_s": round(rate, 1)}), flush=True)
+        for l in (20, 24, 27, 30):
+            ms, rate = fa.probe_bench(fa.PROBE_UPLOAD, l, steps=max(4, 1 << (31 - l)))
+            print(json.dumps({"kind": fa.PROBE_KIND_NAMES[fa.PROBE_UPLOAD], "buffer_bytes": 1 << l,
                               "ms": round(ms, 4), "bytes_per_s": round(rate, 1)}), flush=True)
         return 0
 

@@ -228,16 +228,19 @@ def gather_bench(log2_bytes: int, steps: int = 64, device: int = 0):
     return ms.value, rate.value
 
 
-PROBE_LOAD, PROBE_LOAD_NONTEMPORAL, PROBE_LOAD_AGENT, PROBE_LOAD_SYSTEM, PROBE_ATOMIC_OR, PROBE_STREAM = range(6)
+(PROBE_LOAD, PROBE_LOAD_NONTEMPORAL, PROBE_LOAD_AGENT, PROBE_LOAD_SYSTEM, PROBE_ATOMIC_OR, PROBE_STREAM,
+ PROBE_UPLOAD) = range(7)
 PROBE_KIND_NAMES = {PROBE_LOAD: "load", PROBE_LOAD_NONTEMPORAL: "non-temporal load",
                     PROBE_LOAD_AGENT: "agent-scope atomic load", PROBE_LOAD_SYSTEM: "system-scope atomic load",
-                    PROBE_ATOMIC_OR: "device-scope atomicOr", PROBE_STREAM: "streaming read+write pass"}
+                    PROBE_ATOMIC_OR: "device-scope atomicOr", PROBE_STREAM: "streaming read+write pass",
+                    PROBE_UPLOAD: "pinned-host upload by the server's upload kernel"}
 
 
 def probe_bench(kind: int, log2_bytes: int, steps: int = 64, device: int = 0):
     """(ms, rate) of random accesses of one kind (PROBE_*) into a
     2^log2_bytes array, in accesses/s; for PROBE_STREAM, `steps` passes that
-    read and write back the whole array, in bytes/s."""
+    read and write back the whole array, in bytes/s; for PROBE_UPLOAD, the
+    time of one upload of a 2^log2_bytes pinned buffer and bytes/s."""
     ms, rate = _c.c_float(0.0), _c.c_double(0.0)
     _check(_lib.ffbf_probe_bench(device, log2_bytes, steps, kind, _c.byref(ms), _c.byref(rate)))
     return ms.value, rate.value

@@ -1293,10 +1293,67 @@ int ffbf_device_count(int *out)
     return FFBF_OK;
 }
 
+namespace {
+
+// FFBF_PROBE_UPLOAD: 64 pieces of a pinned buffer, piece i starting i mod 16
+// bytes into its 1/64 share and 16 bytes shorter than it, uploaded back to
+// back into a device buffer by k_upload, once to warm and `steps` times timed.
+int upload_bench(uint32_t log2_bytes, uint32_t steps, float *ms, double *bytes_per_s)
+{
+    if (log2_bytes < 16 || log2_bytes > 31) return fail(FFBF_EINVAL, "upload bench: log2_bytes must be 16..31");
+    const uint64_t bytes = 1ull << log2_bytes, share = bytes / ffbf::kUploadMax;
+    uint8_t *h = nullptr, *d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int st) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (h) (void)hipHostFree(h);
+        if (d) (void)hipFree(d);
+        return st;
+    };
+    if (hipHostMalloc((void **)&h, bytes, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return done(fail(FFBF_ENOMEM, "upload bench buffers"));
+    }
+    std::memset(h, 0x5A, bytes);
+    ffbf::UploadArgs ua{};
+    ua.dst = d;
+    uint64_t at = 0, total = 0;
+    for (uint32_t i = 0; i < ffbf::kUploadMax; i++) {
+        ua.src[i] = (uint64_t)(uintptr_t)(h + (uint64_t)i * share + (i & 15u));
+        ua.len[i] = share - 16u;
+        ua.dst_off[i] = at;
+        ua.seg_start[i] = ua.total_segs;
+        ua.total_segs += ffbf::upload_segs(at, ua.len[i]);
+        at += ua.len[i];
+        total += ua.len[i];
+    }
+    ua.n = ffbf::kUploadMax;
+    ua.seg_start[ua.n] = ua.total_segs;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return done(fail(FFBF_EDEVICE, "event creation"));
+    bool ok = ffbf::launch_upload(ua, nullptr) == hipSuccess && hipEventRecord(e0, nullptr) == hipSuccess;
+    for (uint32_t s = 0; ok && s < steps; s++) ok = ffbf::launch_upload(ua, nullptr) == hipSuccess;
+    ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    if (!ok) return done(fail(FFBF_EDEVICE, std::string("upload bench: ") + hipGetErrorString(hipGetLastError())));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / (float)steps;
+    if (bytes_per_s) *bytes_per_s = (double)total * (double)steps / ((double)t * 1e-3);
+    return done(FFBF_OK);
+}
+
+}  // namespace
+
 int ffbf_probe_bench(int device, uint32_t log2_bytes, uint32_t steps, uint32_t kind, float *ms, double *ops_per_s)
 {
     if (log2_bytes < 4 || log2_bytes > 34 || steps == 0) return fail(FFBF_EINVAL, "log2_bytes must be 4..34, steps > 0");
-    if (kind > FFBF_PROBE_STREAM) return fail(FFBF_EINVAL, "unknown probe kind");
+    if (kind > FFBF_PROBE_UPLOAD) return fail(FFBF_EINVAL, "unknown probe kind");
+    if (kind == FFBF_PROBE_UPLOAD) {
+        HIP_TRY(hipSetDevice(device));
+        return upload_bench(log2_bytes, steps, ms, ops_per_s);
+    }
     HIP_TRY(hipSetDevice(device));
     int n_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));

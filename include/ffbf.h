@@ -432,16 +432,21 @@ int ffbf_gather_bench(int device, uint32_t log2_bytes, uint32_t steps, float *ms
  * or system scope, or device-scope atomicOr with the result unused (the
  * filter build's and the feed-forward's operation); or, for comparison,
  * `steps` streaming passes that read and write back the whole array.
+ * FFBF_PROBE_UPLOAD times `steps` uploads of a 2^log2_bytes pinned host
+ * buffer (log2_bytes 16..31) into device memory by the server's upload
+ * kernel, as 64 pieces at staggered source alignments packed back to back.
  * Reports one launch's time and accesses per second (bytes read and written
- * per second for the streaming passes).  No counterpart in the reference:
- * it measures the rates that bound the scan and the build. */
+ * per second for the streaming passes, bytes uploaded per second for the
+ * upload).  No counterpart in the reference: it measures the rates that
+ * bound the scan, the build and the serving loop's uploads. */
 enum {
     FFBF_PROBE_LOAD = 0,
     FFBF_PROBE_LOAD_NONTEMPORAL = 1,
     FFBF_PROBE_LOAD_AGENT = 2,
     FFBF_PROBE_LOAD_SYSTEM = 3,
     FFBF_PROBE_ATOMIC_OR = 4,
-    FFBF_PROBE_STREAM = 5
+    FFBF_PROBE_STREAM = 5,
+    FFBF_PROBE_UPLOAD = 6
 };
 int ffbf_probe_bench(int device, uint32_t log2_bytes, uint32_t steps, uint32_t kind, float *ms, double *ops_per_s);
 

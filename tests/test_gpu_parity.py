@@ -268,6 +268,13 @@ def test_scan_configs_agree_on_one_input(W):
 def test_gather_bench_runs():
     ms, rate = fa.gather_bench(20, steps=4)
     assert ms > 0 and rate > 1e9
+    for kind in range(fa.PROBE_UPLOAD + 1):
+        ms, rate = fa.probe_bench(kind, 20, steps=4)
+        assert ms > 0 and rate > 1e9, fa.PROBE_KIND_NAMES[kind]
+    for bad in ((fa.PROBE_UPLOAD, 15), (fa.PROBE_UPLOAD, 32), (fa.PROBE_UPLOAD + 1, 20)):
+        with pytest.raises(fa.FFBFError) as err:
+            fa.probe_bench(bad[0], bad[1], steps=1)
+        assert err.value.status == fa.EINVAL
 
 
 @pytest.mark.parametrize("mode", [fa.MODE_LDS, fa.MODE_GLOBAL])
