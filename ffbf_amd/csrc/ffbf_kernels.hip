@@ -492,13 +492,18 @@ __global__ __launch_bounds__(256) void k_query_lines(QueryArgs a)
 // the same shift r within a piece) and lane 63 only loads.  Source reads
 // are buffer loads over the piece's 16-aligned extent: a chunk with no byte
 // of the piece is out of the buffer's range and reads as zeros without an
-// access, so no read leaves the pages the piece lies in.  They are
-// system-scope (sc0 sc1) loads, so the host's latest bytes are read, never
-// a cached copy.  Chunks cut by a piece's ends are written byte by byte,
-// which leaves the neighbouring piece's bytes of a shared chunk alone.
+// access, so no read leaves the pages the piece lies in.  Each wave first
+// takes a system-scope acquire (an invalidate of the host lines in its L1
+// and its XCD's L2), so a pinned buffer the host rewrote after an earlier
+// upload is read anew; the text is not written while its request is in
+// flight, so plain loads after that read the host's bytes (system-scope
+// loads instead cost ~11% of the upload rate).  Chunks cut by a piece's
+// ends are written byte by byte, which leaves the neighbouring piece's
+// bytes of a shared chunk alone.
 __global__ __launch_bounds__(256) void k_upload(const UploadArgs a)
 {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // system scope: buffer_inv sc0 sc1
     const uint32_t lane = threadIdx.x & 63u;
     // wave-uniform: the segment search and every per-piece value stay scalar
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -520,7 +525,7 @@ __global__ __launch_bounds__(256) void k_upload(const UploadArgs a)
         // chunk before base wraps to a large offset: out of range as well)
         const uint32_t off = (uint32_t)(16ull * c + (delta - r) - base);
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)extent, 0x00020000);
-        const u32x4 cur = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 17);  // sc0 sc1
+        const u32x4 cur = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
         u32x4 nxt;
         nxt.x = __shfl_down(cur.x, 1);
         nxt.y = __shfl_down(cur.y, 1);
@@ -588,8 +593,8 @@ hipError_t launch_upload(const UploadArgs &a, hipStream_t s)
     if (a.n == 0 || a.n > kUploadMax || a.seg_start[a.n] != a.total_segs) return hipErrorInvalidValue;
     for (uint32_t p = 0; p < a.n; p++)  // one 32-bit buffer range per piece
         if (!a.len[p] || a.len[p] > (1ull << 31)) return hipErrorInvalidValue;
-    // 32-64 workgroups read the link at its full rate (57 GB/s, the same as
-    // the DMA engine); fewer for small batches
+    // 64 workgroups read the link at ~55 GB/s alone (the DMA engine: 57);
+    // fewer for small batches
     const uint32_t blocks = std::min<uint32_t>(64u, (a.total_segs + 3u) / 4u);
     hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();

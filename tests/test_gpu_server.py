@@ -163,6 +163,14 @@ def test_server_upload_kernel():
     with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=2, max_batch_bytes=2 << 20,
                    upload=fa.UPLOAD_KERNEL) as srv:
         for rep in range(2):
+            if rep == 1:
+                # the host rewrites the same pinned bytes with other text: the
+                # second uploads must read the new bytes, not lines the first
+                # left in a cache
+                for i, (f, a, n) in enumerate(reqs):
+                    _, b, off, _, _ = filters[f]
+                    pin.array[a:a + n] = dense_text(b, off, 600 + i, n)
+                expect = [alone(filters, f, pin.array[a:a + n].copy()) for f, a, n in reqs]
             ids = [srv.submit(f, pin.array[a:a + n]) for f, a, n in reqs]
             for i, rid in enumerate(ids):
                 got, _ = srv.wait(rid)
