@@ -7,7 +7,8 @@
 // of max_batch_bytes.  The scheduler hands a free slot, in arrival order,
 // every pending piece of request text that still fits its buffer and the
 // server's HBM budget; the slot copies the pieces into its buffer straight
-// from the requests' memory, runs one batched scan (ffbf_batch_run) and
+// from the requests' memory (pieces adjacent in host memory in one copy),
+// runs one batched scan (ffbf_batch_run) and
 // returns each piece's matches.  Requests that arrive while slots are busy
 // wait and are admitted by the next slot to free up; several slots (four by
 // default) work at once, so one batch's copies overlap another's scan.  A
@@ -137,6 +138,20 @@ class GpuExecutor : public SchedExecutor {
             err = std::string("text upload kernel: ") + hipGetErrorString(e);
             return false;
         };
+        // DMA copies of pieces adjacent in host memory (a caller's receive
+        // ring, consecutive requests of one buffer) are merged: they are
+        // adjacent in the slot's buffer too, and each copy command has a
+        // fixed cost
+        const uint8_t *run_src = nullptr;
+        uint64_t run_at = 0, run_n = 0;
+        auto flush_run = [&]() -> bool {
+            if (!run_n) return true;
+            const hipError_t e = hipMemcpyAsync(sl.d_text + run_at, run_src, run_n, hipMemcpyHostToDevice, stream);
+            run_n = 0;
+            if (e == hipSuccess) return true;
+            err = std::string("text upload: ") + hipGetErrorString(e);
+            return false;
+        };
         for (size_t i = 0; i < work.size(); i++) {
             const uint64_t n = work[i]->n, at = sl.offsets[i];
             if (n && upload_kernel && work[i]->device_readable) {
@@ -147,15 +162,20 @@ class GpuExecutor : public SchedExecutor {
                 ua.total_segs += upload_segs(at, n);
                 by_kernel += n;
                 if (++ua.n == kUploadMax && !flush()) return FFBF_EDEVICE;
-            } else if (n && hipMemcpyAsync(sl.d_text + at, work[i]->text, n, hipMemcpyHostToDevice, stream) !=
-                                hipSuccess) {
-                err = std::string("text upload: ") + hipGetErrorString(hipGetLastError());
-                return FFBF_EDEVICE;
+            } else if (n) {
+                if (run_n && run_src + run_n == work[i]->text && run_at + run_n == at) {
+                    run_n += n;
+                } else {
+                    if (!flush_run()) return FFBF_EDEVICE;
+                    run_src = work[i]->text;
+                    run_at = at;
+                    run_n = n;
+                }
             }
             sl.offsets[i + 1] = at + n;
             sl.route[i] = work[i]->filter;
         }
-        if (!flush()) return FFBF_EDEVICE;
+        if (!flush() || !flush_run()) return FFBF_EDEVICE;
         kernel_bytes += by_kernel;
         uint64_t nm = 0;
         int st = ffbf_batch_set_texts(sl.batch, sl.d_text, sl.offsets.data(), sl.route.data(), (uint32_t)work.size());

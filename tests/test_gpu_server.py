@@ -228,6 +228,42 @@ def test_server_upload_kernel():
         f[0].close()
 
 
+def test_server_adjacent_requests_merged_copies():
+    """Requests back to back in one buffer (pinned and pageable), as a
+    receive ring holds them: the DMA copies of adjacent pieces in a batch are
+    merged.  A request longer than a slot sits among them, its overlapping
+    pieces never adjacent to each other.  Matches equal each alone."""
+    say("server adjacent requests")
+    sbox = fa.sbox_from_seed(95)
+    filters = make_filters(sbox)[:2]
+    rng = np.random.default_rng(95)
+    sizes = [int(x) for x in rng.integers(1, 150_000, size=60)]
+    sizes[30] = 3 << 20  # pieces of a 2 MiB slot
+    total = sum(sizes)
+    pin = fa.PinnedBuffer(total)
+    reqs, at = [], 0
+    for i, n in enumerate(sizes):
+        f = i % 2
+        _, b, off, _, _ = filters[f]
+        pin.array[at:at + n] = dense_text(b, off, 300 + i, n)
+        reqs.append((f, at, n))
+        at += n
+    page = pin.array.copy()  # the same ring in pageable memory
+    expect = [alone(filters, f, pin.array[a:a + n].copy()) for f, a, n in reqs]
+    assert sum(e.size for e in expect) > 10_000
+    for buf in (pin.array, page):
+        with fa.Server([f[0] for f in filters], hbm_budget_bytes=16 << 20, slots=2, max_batch_bytes=2 << 20) as srv:
+            ids = srv.submit_many([(f, buf[a:a + n]) for f, a, n in reqs])
+            for i, rid in enumerate(ids):
+                got, _ = srv.wait(rid)
+                assert np.array_equal(got["pattern_id"], expect[i]["pattern_id"]), (i, reqs[i])
+                assert np.array_equal(got["offset"], expect[i]["offset"]), (i, reqs[i])
+            assert srv.stats()["batches"] < len(reqs)
+    pin.free()
+    for f in filters:
+        f[0].close()
+
+
 def test_server_admission_limits():
     say("server admission")
     sbox = fa.sbox_from_seed(92)
