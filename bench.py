@@ -216,7 +216,7 @@ def run_batched(fa, name, steps=10, per_text_reps=2, scale=1.0, cpu=False):
     }
 
 
-def serve_inputs(fa, n_requests, seed=51, forced=None):
+def serve_inputs(fa, n_requests, seed=51, forced=None, gap=0):
     """Serving workload: 8 resident pattern sets, n log-spaced 10k..2M
     (lengths 24-80; sets alternate DNA and printable ASCII), W=24, k=3,
     filter bits = 16 n rounded up to a power of two (the multi-filter rule);
@@ -240,12 +240,14 @@ def serve_inputs(fa, n_requests, seed=51, forced=None):
     route = [int(fa.gen_u64(seed, 23, i) % 8) for i in range(n_requests)]
     # request texts live in page-locked host memory, as a serving frontend's
     # receive buffers would
-    pinned = fa.PinnedBuffer(int(sizes.sum()))
+    # (gap > 0: that many bytes between consecutive requests, as requests in
+    # separate buffers are)
+    pinned = fa.PinnedBuffer(int(sizes.sum()) + gap * n_requests)
     reqs, pos = [], 0
     for i in range(n_requests):
         pb, po, _, alpha = sets[route[i]]
         t = pinned.array[pos:pos + int(sizes[i])]
-        pos += int(sizes[i])
+        pos += int(sizes[i]) + gap
         t[:] = fa.gen_bytes(alpha, seed * 1000 + 1 + route[i], int(sizes[i]), start=int(fa.gen_u64(seed, 24, i) >> 24))
         fa.plant(seed * 1000 + 9 + i, (t.size + (1 << 20) - 1) >> 20, t, pb, po)
         reqs.append((route[i], t))
@@ -265,7 +267,8 @@ def percentile(xs, q):
 UPLOADS = {"default": 0, "copy": 1, "kernel": 2}
 
 
-def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20, reps=5, upload=0):
+def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch_bytes=128 << 20, reps=5, upload=0,
+                gap=0):
     """Continuous-batching server vs the plain per-request path (one
     ffbf_match_text per request, first come first served) on the same
     requests.  Saturation throughput of each (everything queued at once);
@@ -281,7 +284,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
     import numpy as np
 
     t0 = time.perf_counter()
-    d = serve_inputs(fa, n_requests)
+    d = serve_inputs(fa, n_requests, gap=gap)
     total = sum(t.size for _, t in d["requests"])
     log(f"serve: {n_requests} requests, {total / 2**30:.2f} GiB, 8 filters, generated in {time.perf_counter() - t0:.1f} s")
     ms = []
@@ -490,6 +493,7 @@ def run_serving(fa, n_requests=600, load=0.7, budget=1 << 30, slots=4, max_batch
         "load_sweep": sweep,
         "batches": int(stats["batches"]),
         "upload": {0: "default", 1: "copy", 2: "kernel"}.get(upload, str(upload)),
+        "request_layout": f"{gap} bytes between requests" if gap else "back to back in one pinned buffer",
         "kernel_upload_bytes": int(stats["kernel_upload_bytes"]),
         "upload_kernel_leg": kernel_leg,
         "hbm_budget_bytes": budget,
@@ -1062,6 +1066,9 @@ def main() -> int:
                          "pinned-host uploads by the upload kernel of 1 MiB..1 GiB) and exit")
     ap.add_argument("--serve-upload", choices=["default", "copy", "kernel"], default="default",
                     help="how the serving leg's request text reaches the device (server upload method)")
+    ap.add_argument("--serve-gap", type=int, default=0, metavar="BYTES",
+                    help="bytes left between consecutive serving requests in their pinned buffer (0: back to "
+                         "back, as in a receive ring; >0: no two requests adjacent, as in separate buffers)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the scan's HBM traffic")
     ap.add_argument("--traffic-child", type=int, default=0, metavar="BYTES", help=argparse.SUPPRESS)
@@ -1156,7 +1163,8 @@ def main() -> int:
         return 0
 
     if args.workload == "serve-8sets":
-        r = run_serving(fa, n_requests=max(64, args.steps * 100), upload=UPLOADS[args.serve_upload])
+        r = run_serving(fa, n_requests=max(64, args.steps * 100), upload=UPLOADS[args.serve_upload],
+                        gap=args.serve_gap)
         print(json.dumps({"metric": METRIC, "value": r["saturation_gbs"], "unit": "GB/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "dtype": "u32",
@@ -1198,7 +1206,7 @@ def main() -> int:
         out["batched_multi_filter"] = run_batched(fa, "batch-2048texts-64sets", steps=5, per_text_reps=1,
                                                   cpu=not args.no_cpu)
         log("serving leg (serve-8sets) ...")
-        out["serving"] = run_serving(fa, n_requests=600, upload=UPLOADS[args.serve_upload])
+        out["serving"] = run_serving(fa, n_requests=600, upload=UPLOADS[args.serve_upload], gap=args.serve_gap)
         log("ingest leg (host memory / file) ...")
         out["ingest"] = run_ingest(fa, reps=2)
         log("split filter leg (split-1m-2g-lines) ...")
