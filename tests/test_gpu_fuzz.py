@@ -85,10 +85,12 @@ def test_pair_method_random(case):
     shift = int(rng.integers(0, 16))
     sbox = fa.sbox_from_seed(int(rng.integers(1, 1 << 30)))
     b, off, text = random_inputs(rng, W, lines=False)
+    build = int(rng.choice([fa.BUILD_AUTO, fa.BUILD_ATOMIC, fa.BUILD_BINNED]))
     say(f"pair case {case}: W={W} k={k} log2={log2} mode={mode} fold={fold} probes={probes} block={block} "
-        f"n_pat={off.size - 1} n={text.size} shift={shift}")
+        f"n_pat={off.size - 1} n={text.size} shift={shift} build={build}")
     r = Oracle(W, k, log2, sbox).run(text, b, off)
     with fa.Matcher(W, k, log2, sbox) as m:
+        m.set_build_method(build)
         m.set_patterns(b, off)
         m.build()
         m.set_scan_mode(mode, fold)
@@ -115,10 +117,12 @@ def test_pair_method_random_beyond_l2(case):
     shift = int(rng.integers(0, 16))
     sbox = fa.sbox_from_seed(int(rng.integers(1, 1 << 30)))
     b, off, text = random_inputs(rng, W, lines=False)
+    build = int(rng.choice([fa.BUILD_AUTO, fa.BUILD_ATOMIC, fa.BUILD_BINNED]))
     say(f"pair case beyond L2 {case}: W={W} k={k} log2={log2} mode={mode} fold={fold} probes={probes} "
-        f"block={block} n_pat={off.size - 1} n={text.size} shift={shift}")
+        f"block={block} n_pat={off.size - 1} n={text.size} shift={shift} build={build}")
     r = Oracle(W, k, log2, sbox).run(text, b, off)
     with fa.Matcher(W, k, log2, sbox) as m:
+        m.set_build_method(build)
         m.set_patterns(b, off)
         m.build()
         m.set_scan_mode(mode, fold)
@@ -145,10 +149,12 @@ def test_split_method_random(case):
     shift = int(rng.integers(0, 16))
     sb = rng.integers(0, 2**64, size=256, dtype=np.uint64)
     b, off, text = random_inputs(rng, W, lines=True)
+    build = int(rng.choice([fa.BUILD_AUTO, fa.BUILD_ATOMIC, fa.BUILD_BINNED]))
     say(f"split case {case}: W={W} k={k} s={s} c={c} e={e} mode={mode} fold={fold} probes={probes} "
-        f"n_pat={off.size - 1} n={text.size} shift={shift}")
+        f"n_pat={off.size - 1} n={text.size} shift={shift} build={build}")
     r = SplitOracle(W, k, s, c, e, sb).run(text, b, off)
     with fa.Matcher.split(W, k, s, c, e, sb) as m:
+        m.set_build_method(build)
         m.set_patterns(b, off)
         m.build()
         m.set_scan_mode(mode, fold)
