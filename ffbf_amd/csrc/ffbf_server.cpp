@@ -121,8 +121,9 @@ class GpuExecutor : public SchedExecutor {
         sl.offsets.assign(work.size() + 1, 0);
         sl.route.resize(work.size());
         const hipStream_t stream = batch_stream(sl.batch);
-        // pieces in device-mapped host memory go by the copy kernel, up to
-        // kUploadMax per launch; the rest by the DMA engine
+        // with FFBF_UPLOAD_KERNEL, pieces in device-mapped host memory go by
+        // the copy kernel, up to kUploadMax per launch; the rest (all pieces
+        // otherwise) by the DMA engine
         UploadArgs &ua = sl.upload;
         ua.dst = sl.d_text;
         ua.n = 0;
