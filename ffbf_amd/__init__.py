@@ -152,6 +152,7 @@ ABI = {
     "ffbf_set_scan_mode": (_i32, [_P, _i32, _u32]),
     "ffbf_set_scan_tuning": (_i32, [_P, _u32, _u32]),
     "ffbf_set_build_method": (_i32, [_P, _i32]),
+    "ffbf_set_timing": (_i32, [_P, _i32]),
     "ffbf_gather_bench": (_i32, [_i32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_probe_bench": (_i32, [_i32, _u32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_get_timings": (_i32, [_P, _c.POINTER(Timings)]),
@@ -467,6 +468,10 @@ class Matcher:
     def set_build_method(self, method: int) -> None:
         """BUILD_AUTO, BUILD_ATOMIC or BUILD_BINNED, from the next build()."""
         _check(_lib.ffbf_set_build_method(self._h, method))
+
+    def set_timing(self, on: bool) -> None:
+        """Stage timing events (default on; each costs ~4-5 us between kernels)."""
+        _check(_lib.ffbf_set_timing(self._h, 1 if on else 0))
 
     # -- scan / pass / verify
     def scan(self, text) -> int:

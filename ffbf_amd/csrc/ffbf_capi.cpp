@@ -48,6 +48,8 @@ int use_device(ffbf_ctx *c)
 
 int record(ffbf_ctx *c, int e)
 {
+    c->ev_recorded[e] = false;
+    if (!c->timing) return FFBF_OK;
     HIP_TRY(hipEventRecord(c->ev[e], c->stream));
     c->ev_recorded[e] = true;
     return FFBF_OK;
@@ -887,6 +889,13 @@ int ffbf_build(ffbf_ctx *c)
     RET_IF(choose_mode(c));
     c->built = true;
     c->scanned = c->passed = c->verified = false;
+    return FFBF_OK;
+}
+
+int ffbf_set_timing(ffbf_ctx *c, int on)
+{
+    RET_IF(check_ctx(c));
+    c->timing = on != 0;
     return FFBF_OK;
 }
 

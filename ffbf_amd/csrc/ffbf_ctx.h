@@ -186,6 +186,7 @@ struct ffbf_ctx {
     bool patterns = false, built = false, scanned = false, passed = false, verified = false;
     hipEvent_t ev[ffbf::EV_N] = {};
     bool ev_recorded[ffbf::EV_N] = {};
+    bool timing = true;  // record the stage events (ffbf_set_timing)
 
     // line queries (ffbf_query_lines): the lines, their offsets and answers
     ffbf::DevBuf<uint8_t> q_bytes, q_hits;

@@ -289,6 +289,13 @@ int ffbf_set_scan_tuning(ffbf_ctx *ctx, uint32_t probes, uint32_t block);
 enum { FFBF_BUILD_AUTO = 0, FFBF_BUILD_ATOMIC = 1, FFBF_BUILD_BINNED = 2 };
 int ffbf_set_build_method(ffbf_ctx *ctx, int method);
 
+/* Stage timing events on (1, the default) or off (0).  Each event recorded
+ * between two dependent kernels costs ~4-5 us of device time, about ten per
+ * build and run; with timing off ffbf_get_timings reports 0 for the stage
+ * times (the mode, fold and probe fields stay valid).  The reference times
+ * its stages with host clocks around synchronised calls (ffbf.cpp:37-51). */
+int ffbf_set_timing(ffbf_ctx *ctx, int on);
+
 int ffbf_get_timings(ffbf_ctx *ctx, ffbf_timings *out);
 int ffbf_synchronize(ffbf_ctx *ctx);
 

@@ -144,6 +144,30 @@ def test_unaligned_device_text(shift):
     buf.free()
 
 
+def test_timing_off_same_results():
+    """Stage events off: the same F, candidates, R, survivors and matches;
+    the stage times read 0 until they are switched back on."""
+    sbox = fa.sbox_from_seed(7)
+    W = 24
+    b, off = fa.gen_patterns(fa.ALPHA_DNA_GC40, 8, 3000, 24, 60)
+    text = fa.gen_bytes(fa.ALPHA_DNA_GC40, 9, 3 << 20)
+    fa.plant(10, 40, text, b, off)
+    r = Oracle(W, 4, 20, sbox).run(text, b, off)
+    with fa.Matcher(W, 4, 20, sbox) as m:
+        m.set_timing(False)
+        m.set_patterns(b, off)
+        m.build()
+        assert_same(gpu_all(m, text), r, m.filter(fa.FILTER_FORWARD))
+        t = m.timings()
+        assert t["build_ms"] == 0 and t["scan_kernel_ms"] == 0 and t["verify_ms"] == 0
+        assert t["scan_mode"] in (fa.MODE_LDS, fa.MODE_FOLD, fa.MODE_GLOBAL)
+        m.set_timing(True)
+        m.build()
+        assert_same(gpu_all(m, text), r, m.filter(fa.FILTER_FORWARD))
+        t = m.timings()
+        assert t["build_ms"] > 0 and t["scan_kernel_ms"] > 0 and t["verify_ms"] > 0
+
+
 def test_edge_sizes():
     sbox = fa.sbox_from_seed(6)
     W = 19
