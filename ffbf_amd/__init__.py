@@ -153,6 +153,7 @@ ABI = {
     "ffbf_set_scan_tuning": (_i32, [_P, _u32, _u32]),
     "ffbf_set_build_method": (_i32, [_P, _i32]),
     "ffbf_set_timing": (_i32, [_P, _i32]),
+    "ffbf_batch_set_timing": (_i32, [_P, _i32]),
     "ffbf_gather_bench": (_i32, [_i32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_probe_bench": (_i32, [_i32, _u32, _u32, _u32, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double)]),
     "ffbf_get_timings": (_i32, [_P, _c.POINTER(Timings)]),
@@ -692,6 +693,10 @@ class Batch:
         ms = _c.c_float(0)
         _check(_lib.ffbf_batch_autotune(self._h, _c.byref(t), _c.byref(ms)))
         return t.as_dict(), ms.value
+
+    def set_timing(self, on: bool) -> None:
+        """Timing events around each run (default on; ~10 us per run)."""
+        _check(_lib.ffbf_batch_set_timing(self._h, 1 if on else 0))
 
     def timings(self):
         run = _c.c_float(0)

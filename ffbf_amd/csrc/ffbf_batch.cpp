@@ -71,6 +71,7 @@ struct ffbf_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t e_run0 = nullptr, e_run1 = nullptr, e_scan0 = nullptr, e_scan1 = nullptr;
     float run_ms = 0.0f, scan_ms = -1.0f;
+    bool timing = true;  // record e_run0/e_run1 around each run (ffbf_batch_set_timing)
 
     uint64_t n_cand = 0, n_match = 0;
     bool ran = false, sorted_on_device = false, sorted_valid = false;
@@ -298,17 +299,20 @@ int run_once(ffbf_batch *b)
         b->graph = g;
         HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
-    HIP_TRY(hipEventRecord(b->e_run0, b->stream));
+    if (b->timing) HIP_TRY(hipEventRecord(b->e_run0, b->stream));
     if (graph) {
         HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     } else {
-        RET_IF(enqueue_run(b, true));
+        RET_IF(enqueue_run(b, b->timing));
     }
-    HIP_TRY(hipEventRecord(b->e_run1, b->stream));
+    if (b->timing) HIP_TRY(hipEventRecord(b->e_run1, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    (void)hipEventElapsedTime(&b->run_ms, b->e_run0, b->e_run1);
+    b->run_ms = 0.0f;
     b->scan_ms = -1.0f;
-    if (!graph) (void)hipEventElapsedTime(&b->scan_ms, b->e_scan0, b->e_scan1);
+    if (b->timing) {
+        (void)hipEventElapsedTime(&b->run_ms, b->e_run0, b->e_run1);
+        if (!graph) (void)hipEventElapsedTime(&b->scan_ms, b->e_scan0, b->e_scan1);
+    }
     return FFBF_OK;
 }
 
@@ -732,6 +736,13 @@ int ffbf_batch_autotune(ffbf_batch *b, ffbf_batch_tuning *best, float *best_ms)
     const ffbf_batch_tuning start = b->tuning;
     ffbf_batch_tuning top = start;
     float top_ms = 1e30f;
+    // the candidates are ranked by their timed runs
+    struct TimingOn {
+        ffbf_batch *b;
+        bool was;
+        ~TimingOn() { b->timing = was; }
+    } timing_on{b, b->timing};
+    b->timing = true;
     // 256 was 2% ahead of 128 on the 64-set batch, 512 and up fell behind
     // (fewer chunks than workgroups can balance); 16 never won
     const uint32_t chunk_opts[] = {32u, 64u, 128u, 256u};
@@ -762,6 +773,13 @@ int ffbf_batch_autotune(ffbf_batch *b, ffbf_batch_tuning *best, float *best_ms)
     RET_IF(ffbf_batch_set_tuning(b, &top));
     if (best) *best = top;
     if (best_ms) *best_ms = top_ms;
+    return FFBF_OK;
+}
+
+int ffbf_batch_set_timing(ffbf_batch *b, int on)
+{
+    RET_IF(check_batch(b));
+    b->timing = on != 0;
     return FFBF_OK;
 }
 

@@ -354,6 +354,10 @@ int ffbf_batch_autotune(ffbf_batch *batch, ffbf_batch_tuning *best, float *best_
 /* Device time of the last run (whole graph) and of its scan kernel (only
  * when the run was not a graph replay; else -1). */
 int ffbf_batch_timings(ffbf_batch *batch, float *run_ms, float *scan_ms);
+/* Timing events around each run on (1, the default) or off (0; about 10 us
+ * less per run, ffbf_batch_timings then reports 0).  The autotuner times
+ * its candidates either way.  A server's batches run without them. */
+int ffbf_batch_set_timing(ffbf_batch *batch, int on);
 
 /* ---- Serving: continuous batching over resident filters ----
  * The reference has no request loop.  A server keeps `slots` copies of the

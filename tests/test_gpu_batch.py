@@ -151,10 +151,18 @@ def test_batch_autotune_and_new_texts():
     expect, _ = oracle_batch(sbox, filters, texts, route)
     with fa.Batch([f[0] for f in filters]) as bt:
         bt.set_texts(dev.ptr, off, route)
+        # the autotuner times its candidates with the run events off too,
+        # and leaves them off
+        bt.set_timing(False)
         best, ms = bt.autotune()
         assert ms > 0 and best["chunk_iters"] in (32, 64, 128, 256)
         bt.run()
         assert as_rows(bt.matches()) == expect
+        assert bt.timings()[0] == 0
+        bt.set_timing(True)
+        bt.run()
+        assert as_rows(bt.matches()) == expect
+        assert bt.timings()[0] > 0
         # a different batch on the same object (graph reused)
         texts2, off2, route2 = make_texts(filters, 25, 8)
         dev2 = fa.DeviceBuffer.from_host(np.concatenate(texts2))
