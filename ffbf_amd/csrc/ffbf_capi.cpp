@@ -814,11 +814,11 @@ int ffbf_set_patterns(ffbf_ctx *c, const uint8_t *bytes, const uint64_t *offsets
     HIP_TRY(hipMemcpyAsync(c->off.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
     // prefix hash pairs only (F untouched until ffbf_build)
     if (c->split)
-        HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->smap,
-                                         nullptr, c->pat_hash.p, c->stream));
+        HIP_TRY(ffbf::launch_prefix_hash_split(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->pat_hash.p,
+                                               c->stream));
     else
-        HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->params.k, c->mask,
-                                   nullptr, c->pat_hash.p, c->stream));
+        HIP_TRY(ffbf::launch_prefix_hash(c->bytes.p, c->off.p, n, c->sbox.p, c->params.window, c->pat_hash.p,
+                                         c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->n_pat = n;
     c->n_long = n_long;
@@ -877,12 +877,8 @@ int ffbf_build(ffbf_ctx *c)
                                           c->stream));
     } else {
         HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
-        if (c->split)
-            HIP_TRY(ffbf::launch_build_split(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
-                                             c->smap, c->F.p, c->pat_hash.p, c->stream));
-        else
-            HIP_TRY(ffbf::launch_build(c->bytes.p, c->off.p, c->n_pat, c->sbox.p, c->params.window, c->params.k,
-                                       c->mask, c->F.p, c->pat_hash.p, c->stream));
+        HIP_TRY(ffbf::launch_build_atomic(c->off.p, c->pat_hash.p, c->n_pat, c->params.window, c->params.k, c->mask,
+                                          c->split, c->smap, c->F.p, c->stream));
     }
     RET_IF(record(c, EV_B1));
     c->f_set_bits = -1;

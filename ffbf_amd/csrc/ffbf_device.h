@@ -237,12 +237,17 @@ inline uint32_t upload_segs(uint64_t dst_off, uint64_t n)
 hipError_t launch_fill32(uint32_t *p, uint64_t n_words, uint32_t value, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 hipError_t launch_upload(const UploadArgs &a, hipStream_t s);
-hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
-                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, hipStream_t s);
-// The split method's build: 64-bit prefix hashes from a 256-entry (lo, hi)
-// sbox, k bits placed by m (F null: hashes only).
-hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
-                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash, hipStream_t s);
+// Prefix hash pairs of the patterns' first W bytes ((0, 0) for shorter ones).
+hipError_t launch_prefix_hash(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
+                              uint2 *pat_hash, hipStream_t s);
+// The split method's: 64-bit prefix hashes from a 256-entry (lo, hi) sbox.
+hipError_t launch_prefix_hash_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
+                                    uint32_t W, uint2 *pat_hash, hipStream_t s);
+// Atomic build: the k forward bits of every pattern at least W long, set by
+// atomicOr into a cleared F from the prefix hash pairs (placed by m when
+// split).
+hipError_t launch_build_atomic(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W, uint32_t k,
+                               uint32_t mask, bool split, const SplitMap &m, uint32_t *F, hipStream_t s);
 // Atomic-free build (filters of more than one 2^20-bit slice): bit indices of
 // kBinEntries / k patterns per block sorted by slice into `staging`
 // (build_bin_blocks(n, k) * kBinEntries words) with per-block slice offsets in

@@ -3,12 +3,13 @@
 //
 // Stages and the reference code whose results they reproduce bit for bit
 // (efficient/ffbf):
-//   k_build         grepSetup        src/gpgpu/greptest.cu:96-123
+//   k_build         prefix hash pairs of grepSetup  src/gpgpu/greptest.cu:96-123
+//   k_build_atomic  the bits grepSetup sets (and, split, Filter::operator<<)
 //   k_pattern_pass  filterPatterns   src/gpgpu/greptest.cu:253-290
 //   k_verify        the exact step the reference leaves to `grep -Ff`
 //                   (src/rabin-karp/README:49-52)
 //   k_fold          OR-fold of F used by the scan as an exact-superset prefilter
-//   k_build_split   Filter::operator<< of the CPU binary  src/rabin-karp/ffbf.cpp:268-283
+//   k_build_split   Filter::operator<<'s prefix hashes  src/rabin-karp/ffbf.cpp:268-283
 //   k_build_bin, k_build_slices  the same bits for filters of several 2^20-bit
 //                   slices, set without global atomics
 #include "ffbf_common.h"
@@ -52,13 +53,12 @@ __device__ __forceinline__ void prefix_bytes(const uint8_t *bytes, uint64_t a0, 
     }
 }
 
-// Filter build (grepSetup, greptest.cu:96-123): one lane per pattern hashes
-// its W-byte prefix and sets k forward bits; patterns shorter than W are
-// skipped (ffbf.cpp:271-275).  The hash pair is kept for the pattern pass and
-// verification; with F null only the hash pairs are computed.
+// Prefix hash pairs, once per pattern set (the hashing half of grepSetup,
+// greptest.cu:96-123): one lane per pattern hashes its W-byte prefix; a
+// pattern shorter than W keeps (0, 0) and sets no bits (ffbf.cpp:271-275).
+// The pairs feed both builds, the pattern pass and verification.
 __global__ __launch_bounds__(256) void k_build(const uint8_t *bytes, const uint64_t *off, uint64_t n,
-                                               const uint32_t *sbox, uint32_t W, uint32_t k, uint32_t mask,
-                                               uint32_t *F, uint2 *pat_hash)
+                                               const uint32_t *sbox, uint32_t W, uint2 *pat_hash)
 {
     __shared__ uint32_t sb[256];
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) sb[i] = sbox[i];
@@ -73,27 +73,18 @@ __global__ __launch_bounds__(256) void k_build(const uint8_t *bytes, const uint6
                 h1 = rotl(h1, 1) ^ s;
                 h2 = rotl(h2, 3) ^ s;
             });
-            if (F) {
-                for (int j = 0; j < 5; j++) {
-                    if (j >= (int)k) break;
-                    const uint32_t idx = fwd_idx(j, h1, h2) & mask;
-                    atomicOr(&F[idx >> 5], 1u << (idx & 31u));
-                }
-            }
         }
         pat_hash[i] = make_uint2(h1, h2);
     }
 }
 
-// The split method's build (Filter::operator<<, ffbf.cpp:268-283): one lane
-// per pattern hashes its W-byte prefix with the 64-bit loop form
-// h = rol64(h,1) ^ S[c] (the ring update of hash_rot_sbox_pre_2.h:41-50 before
-// the window is full; the sbox has bytes >= 0x80 mapped to entry 0) and sets
-// k bits, each in the cache or external part (setBitsInFilter,
-// ffbf.cpp:196-203).  Hash halves are kept as (h1, h2) = (low, high).
+// The split method's prefix hashes (the hashing half of Filter::operator<<,
+// ffbf.cpp:268-283): one lane per pattern hashes its W-byte prefix with the
+// 64-bit loop form h = rol64(h,1) ^ S[c] (the ring update of
+// hash_rot_sbox_pre_2.h:41-50 before the window is full; the sbox has bytes
+// >= 0x80 mapped to entry 0).  Hash halves are kept as (h1, h2) = (low, high).
 __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n,
-                                                     const uint2 *sbox, uint32_t W, uint32_t k, SplitMap m,
-                                                     uint32_t *F, uint2 *pat_hash)
+                                                     const uint2 *sbox, uint32_t W, uint2 *pat_hash)
 {
     __shared__ uint2 sb[256];
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) sb[i] = sbox[i];
@@ -110,15 +101,29 @@ __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const
                 lo = rl ^ v.x;
                 hi = rh ^ v.y;
             });
-            if (F) {
-                for (int j = 0; j < kSplitMaxK; j++) {
-                    if (j >= (int)k) break;
-                    const uint32_t idx = split_place(j, split_fwd_fn(j, lo, hi), m);
-                    atomicOr(&F[idx >> 5], 1u << (idx & 31u));
-                }
-            }
         }
         pat_hash[i] = make_uint2(lo, hi);
+    }
+}
+
+// The atomic build (grepSetup, greptest.cu:96-123; Filter::operator<< with
+// setBitsInFilter, ffbf.cpp:196-203 and 268-283, for the split method): one
+// lane per pattern at least W long sets its k forward bits in a cleared F
+// with atomicOr, from the prefix hash pair computed when the patterns were
+// set; shorter patterns set none (ffbf.cpp:271-275).
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_build_atomic(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W,
+                                                      uint32_t k, uint32_t mask, SplitMap m, uint32_t *F)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (off[i + 1] - off[i] < W) continue;
+        const uint2 h = pat_hash[i];
+        for (int j = 0; j < (SPLIT ? kSplitMaxK : 5); j++) {
+            if (j >= (int)k) break;
+            const uint32_t x = SPLIT ? split_place(j, split_fwd_fn(j, h.x, h.y), m) : (fwd_idx(j, h.x, h.y) & mask);
+            atomicOr(&F[x >> 5], 1u << (x & 31u));
+        }
     }
 }
 
@@ -132,7 +137,7 @@ __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const
 // k_build_slices then gives each slice one workgroup, which sets that slice's
 // bits from every block in an LDS copy (LDS atomics) and writes the slice
 // whole, clearing what no pattern sets.  Bit placement is the same as
-// k_build's and k_build_split's.
+// k_build_atomic's.
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_build_bin(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t P,
                                                    uint32_t W, uint32_t k, uint32_t mask, SplitMap m, uint32_t ns,
@@ -600,24 +605,39 @@ hipError_t launch_upload(const UploadArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_build(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
-                        uint32_t k, uint32_t mask, uint32_t *F, uint2 *pat_hash, hipStream_t s)
+hipError_t launch_prefix_hash(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox, uint32_t W,
+                              uint2 *pat_hash, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255u) / 256u;
     if (blocks > 8192u) blocks = 8192u;
-    hipLaunchKernelGGL(k_build, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, sbox, W, k, mask, F, pat_hash);
+    hipLaunchKernelGGL(k_build, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, sbox, W, pat_hash);
     return hipGetLastError();
 }
 
-hipError_t launch_build_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
-                              uint32_t W, uint32_t k, SplitMap m, uint32_t *F, uint2 *pat_hash, hipStream_t s)
+hipError_t launch_prefix_hash_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
+                                    uint32_t W, uint2 *pat_hash, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255u) / 256u;
     if (blocks > 8192u) blocks = 8192u;
     hipLaunchKernelGGL(k_build_split, dim3((uint32_t)blocks), dim3(256), 0, s, bytes, off, n, (const uint2 *)sbox64,
-                       W, k, m, F, pat_hash);
+                       W, pat_hash);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_atomic(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W, uint32_t k,
+                               uint32_t mask, bool split, const SplitMap &m, uint32_t *F, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    if (split)
+        hipLaunchKernelGGL(k_build_atomic<true>, dim3((uint32_t)blocks), dim3(256), 0, s, off, pat_hash, n, W, k, mask,
+                           m, F);
+    else
+        hipLaunchKernelGGL(k_build_atomic<false>, dim3((uint32_t)blocks), dim3(256), 0, s, off, pat_hash, n, W, k,
+                           mask, m, F);
     return hipGetLastError();
 }
 
@@ -710,62 +730,96 @@ hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s)
 
 namespace {
 
-// Exclusive prefix of the region counts (one workgroup of 1024): offsets[w] =
-// list length before region w, offsets[n] = where the spill goes,
-// offsets[n+1] = spill entries kept.  Counts and the spill counter are
-// zeroed for the next launch; C_CAND becomes the new length, or exceeds
-// out_cap when entries were lost (the caller then grows and rescans).
+// Exclusive prefix of the region counts, by one workgroup of 16 waves:
+// offsets[w] = list length before region w, offsets[n] = where the spill
+// goes, offsets[n+1] = spill entries kept.  Each wave owns a contiguous run
+// of the counts, a whole number of 256-count rounds (one 16-byte group per
+// lane per round), loads it coalesced and sums it; after one barrier it
+// writes its offsets from the totals of the waves before it, by a lane
+// prefix per round.  Counts and the spill counter are zeroed for the next
+// launch; C_CAND becomes the new length, or exceeds out_cap when entries
+// were lost (the caller then grows and rescans).  Only thread 0 reads and
+// writes the counters.  A region keeps at most 4096 entries, so a wave's
+// sums fit 32 bits for n < 2^20.
+constexpr uint32_t kOffWaves = 16;
+constexpr uint32_t kOffHeld = 8;  // rounds a wave keeps in registers between its two passes
+
+// counts[i..i+3] (zeros past n); i % 4 == 0 and counts is 256-byte aligned
+__device__ __forceinline__ uint4 count_group(const uint32_t *counts, uint32_t i, uint32_t n)
+{
+    if (i + 3u < n) return *(const uint4 *)(counts + i);
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (i < n) v.x = counts[i];
+    if (i + 1u < n) v.y = counts[i + 1u];
+    if (i + 2u < n) v.z = counts[i + 2u];
+    return v;
+}
+
 __global__ __launch_bounds__(1024) void k_cand_offsets(uint32_t *counts, uint64_t *offsets, uint32_t n, uint64_t spill_cap,
                                                       uint64_t out_cap, unsigned long long *counters)
 {
-    __shared__ uint64_t s_part[1024];
-    const uint32_t t = threadIdx.x;
-    // 4-count groups per thread, so that counts move as 16-byte vectors
-    const uint32_t per = ((n + 1023u) / 1024u + 3u) & ~3u;
-    const uint32_t lo = t * per, hi = lo + per < n ? lo + per : n;
-    const bool vec = (n & 3u) == 0u;  // then lo, hi are multiples of 4 (counts is 256-byte aligned)
-    uint64_t sum = 0;
-    if (vec) {
-#pragma unroll 4
-        for (uint32_t i = lo; i < hi; i += 4) {
-            const uint4 v = *(const uint4 *)(counts + i);
-            sum += (uint64_t)v.x + v.y + v.z + v.w;
-        }
-    } else {
-        for (uint32_t i = lo; i < hi; i++) sum += counts[i];
+    __shared__ uint32_t s_wave[kOffWaves];
+    __shared__ uint64_t s_base;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = threadIdx.x >> 6;
+    // the scan kernel before this one left the counters final: thread 0
+    // loads all three up front, beside the counts
+    uint64_t base = 0, spill = 0, cvalid = 0;
+    if (threadIdx.x == 0) {
+        base = counters[C_CAND];
+        spill = counters[C_SPILL];
+        cvalid = counters[C_CVALID];
+        s_base = base;
     }
-    s_part[t] = sum;
+    const uint32_t rounds = (n + kOffWaves * 256u - 1u) / (kOffWaves * 256u);
+    const uint32_t first = w * rounds * 256u + lane * 4u;
+    uint4 held[kOffHeld];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kOffHeld; r++) {
+        held[r] = r < rounds ? count_group(counts, first + r * 256u, n) : make_uint4(0u, 0u, 0u, 0u);
+        sum += held[r].x + held[r].y + held[r].z + held[r].w;
+    }
+    for (uint32_t r = kOffHeld; r < rounds; r++) {
+        const uint4 v = count_group(counts, first + r * 256u, n);
+        sum += v.x + v.y + v.z + v.w;
+    }
+    for (uint32_t d = 32u; d; d >>= 1) sum += __shfl_xor(sum, (int)d);
+    if (lane == 0) s_wave[w] = sum;
     __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint64_t v = t >= d ? s_part[t - d] : 0u;
-        __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
+    uint64_t run = s_base, total = 0;
+    for (uint32_t j = 0; j < kOffWaves; j++) {
+        const uint32_t t = s_wave[j];
+        if (j < w) run += t;
+        total += t;
     }
-    const uint64_t base = counters[C_CAND];
-    uint64_t run = base + (t ? s_part[t - 1] : 0u);
-    if (vec) {
-#pragma unroll 4
-        for (uint32_t i = lo; i < hi; i += 4) {
-            const uint4 v = *(const uint4 *)(counts + i);
-            offsets[i] = run;
-            offsets[i + 1] = run + v.x;
-            offsets[i + 2] = run + v.x + v.y;
-            offsets[i + 3] = run + v.x + v.y + v.z;
-            run += (uint64_t)v.x + v.y + v.z + v.w;
+    // one round: the lanes' exclusive prefix over their groups, four offsets
+    // per lane, the counts zeroed; run moves on by the round's total
+    auto emit = [&](uint32_t i, const uint4 v) {
+        const uint32_t g = v.x + v.y + v.z + v.w;
+        uint32_t inc = g;
+        for (uint32_t d = 1u; d < 64u; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d);
+            if (lane >= d) inc += o;
+        }
+        const uint64_t o0 = run + (inc - g), o1 = o0 + v.x, o2 = o1 + v.y, o3 = o2 + v.z;
+        if (i + 3u < n) {
+            *(ulonglong2 *)(offsets + i) = make_ulonglong2(o0, o1);
+            *(ulonglong2 *)(offsets + i + 2u) = make_ulonglong2(o2, o3);
             *(uint4 *)(counts + i) = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            if (i < n) offsets[i] = o0, counts[i] = 0u;
+            if (i + 1u < n) offsets[i + 1u] = o1, counts[i + 1u] = 0u;
+            if (i + 2u < n) offsets[i + 2u] = o2, counts[i + 2u] = 0u;
         }
-    } else {
-        for (uint32_t i = lo; i < hi; i++) {
-            offsets[i] = run;
-            run += counts[i];
-            counts[i] = 0u;
-        }
-    }
-    __syncthreads();
-    if (t == 0) {
-        const uint64_t end_regions = base + s_part[1023];
-        const uint64_t spill = counters[C_SPILL];
+        run += __shfl(inc, 63);
+    };
+#pragma unroll
+    for (uint32_t r = 0; r < kOffHeld; r++)
+        if (r < rounds) emit(first + r * 256u, held[r]);
+    for (uint32_t r = kOffHeld; r < rounds; r++) emit(first + r * 256u, count_group(counts, first + r * 256u, n));
+    if (threadIdx.x == 0) {
+        const uint64_t end_regions = base + total;
         const uint64_t kept = spill < spill_cap ? spill : spill_cap;
         offsets[n] = end_regions;
         offsets[n + 1] = kept;
@@ -775,7 +829,7 @@ __global__ __launch_bounds__(1024) void k_cand_offsets(uint32_t *counts, uint64_
         // the list is whole up to the last entry written, if it was whole
         // before this launch; a lost spill leaves a gap after it
         const uint64_t written = end_regions + kept < out_cap ? end_regions + kept : out_cap;
-        if (counters[C_CVALID] == base) counters[C_CVALID] = written;
+        if (cvalid == base) counters[C_CVALID] = written;
     }
 }
 
