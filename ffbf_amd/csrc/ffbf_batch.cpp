@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "ffbf_ctx.h"
+#include "ffbf_hostsort.h"
 
 using namespace ffbf;
 
@@ -652,11 +653,13 @@ int ffbf_batch_get_matches(ffbf_batch *b, ffbf_batch_match *out, uint64_t cap, u
                                    hipMemcpyDeviceToHost, b->stream));
             HIP_TRY(hipStreamSynchronize(b->stream));
         }
-        if (!b->sorted_on_device)
-            std::sort(b->sorted.begin(), b->sorted.end(), [](const BatchMatchRec &x, const BatchMatchRec &y) {
-                if (x.text != y.text) return x.text < y.text;
-                return x.id != y.id ? x.id < y.id : x.offset < y.offset;
-            });
+        if (!b->sorted_on_device) {
+            // by (text, pattern id, offset): stable passes, least significant field first
+            std::vector<BatchMatchRec> tmp;
+            ffbf::sort_by_major_offset(b->sorted.data(), b->sorted.size(), [](const BatchMatchRec &r) { return r.id; },
+                                       tmp);
+            ffbf::radix_pass_sort(b->sorted.data(), b->sorted.size(), [](const BatchMatchRec &r) { return r.text; }, tmp);
+        }
         b->sorted_valid = true;
     }
     for (uint64_t i = 0; i < b->n_match; i++) {

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "ffbf_ctx.h"
+#include "ffbf_hostsort.h"
 
 using ffbf::Cand;
 using ffbf::MatchRec;
@@ -621,9 +622,8 @@ int load_sorted_matches(ffbf_ctx *c)
         std::memcpy(c->sorted.data(), c->h_match, c->n_match * sizeof(MatchRec));
     }
     if (!c->sorted_on_device)
-        std::sort(c->sorted.begin(), c->sorted.end(), [](const MatchRec &a, const MatchRec &b) {
-            return a.id != b.id ? a.id < b.id : a.offset < b.offset;
-        });
+        ffbf::sort_by_major_offset(c->sorted.data(), c->sorted.size(), [](const MatchRec &r) { return r.id; },
+                                   c->sort_tmp);
     c->sorted_valid = true;
     return FFBF_OK;
 }

@@ -175,6 +175,7 @@ struct ffbf_ctx {
     bool sorted_on_device = false;
     uint64_t n_cand = 0, n_surv = 0, n_match = 0;
     std::vector<ffbf::MatchRec> sorted;
+    std::vector<ffbf::MatchRec> sort_tmp;  // the host sort's second buffer
     bool sorted_valid = false;
     // pinned landing area of the match list on its way to the host: the
     // fused run copies the first kMatchPrefetch records there before its one

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 
+#include "ffbf_hostsort.h"
+
 namespace ffbf {
 
 Scheduler::Scheduler(const SchedConfig &cfg, SchedExecutor *exec) : cfg_(cfg), exec_(exec) {}
@@ -91,9 +93,9 @@ void Scheduler::finish_locked(SchedRequest *r)
         r->error = "request cancelled";
         r->matches.clear();
     } else if (r->status == FFBF_OK) {
-        std::sort(r->matches.begin(), r->matches.end(), [](const ffbf_match &a, const ffbf_match &b) {
-            return a.pattern_id != b.pattern_id ? a.pattern_id < b.pattern_id : a.offset < b.offset;
-        });
+        std::vector<ffbf_match> tmp;
+        sort_by_major_offset(r->matches.data(), r->matches.size(), [](const ffbf_match &m) { return m.pattern_id; },
+                             tmp);
     } else {
         r->matches.clear();
     }

@@ -5,6 +5,8 @@
   and HBM budget, an executor that fails a batch, cancellation before, during
   and after a run, draining at shutdown -- under AddressSanitizer +
   UndefinedBehaviorSanitizer and under ThreadSanitizer.
+* The host radix sort of match records (tests/c/hostsort_test.cpp) against
+  std::stable_sort, under AddressSanitizer + UndefinedBehaviorSanitizer.
 * The C oracle (tests/c/oracle_sanitize.c drives every function of
   oracle/ffbf_oracle.c over W 1..64, empty inputs and short caps) under
   AddressSanitizer + UndefinedBehaviorSanitizer.
@@ -36,6 +38,16 @@ def test_scheduler_under_sanitizers(tmp_path, san):
         flags.append("-fno-sanitize-recover=all")
     subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-Wall", "-Wextra", "-pthread", *flags, "-I", INC, "-I", CSRC,
                     SCHED_TEST, os.path.join(CSRC, "ffbf_sched.cpp"), "-o", exe],
+                   check=True, capture_output=True, text=True)
+    assert "all checks passed" in _run(exe, 300)
+
+
+def test_host_match_sort_under_asan_ubsan(tmp_path):
+    """The radix sort that orders match lists on the host (tests/c/hostsort_test.cpp)."""
+    exe = str(tmp_path / "hostsort_test")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-Wall", "-Wextra", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "c", "hostsort_test.cpp"), "-o", exe],
                    check=True, capture_output=True, text=True)
     assert "all checks passed" in _run(exe, 300)
 
