@@ -40,7 +40,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 
 OK = 0
 EINVAL, EDEVICE, ENOMEM, ESTATE, ECAPACITY, ECANCELED = -1, -2, -3, -4, -5, -6
-FILTER_FORWARD, FILTER_REVERSE = 0, 1
+FILTER_FORWARD, FILTER_REVERSE, FILTER_FOLD, FILTER_L2_FOLD = 0, 1, 2, 3
 MODE_AUTO, MODE_GLOBAL, MODE_FOLD, MODE_LDS, MODE_GFOLD, MODE_HYBRID = -1, 0, 1, 2, 3, 4
 BUILD_AUTO, BUILD_ATOMIC, BUILD_BINNED = 0, 1, 2
 ALPHA_ASCII, ALPHA_DNA_GC40 = 0, 1
@@ -594,8 +594,14 @@ class Matcher:
         return out[:words]
 
     def filter(self, which: int = FILTER_FORWARD) -> np.ndarray:
-        out = np.zeros(self.n_words, dtype=np.uint32)
-        _check(_lib.ffbf_get_filter(self._h, which, _ptr(out), self.n_words))
+        """F, R, or one of the scan's folds (FILTER_FOLD, FILTER_L2_FOLD)."""
+        words = self.n_words
+        if which in (FILTER_FOLD, FILTER_L2_FOLD):
+            t = self.timings()
+            bits = t["fold_log2_bits"] if which == FILTER_FOLD else t["l2_fold_log2_bits"]
+            words = max(1, (1 << bits) // 32) if bits else 1
+        out = np.zeros(words, dtype=np.uint32)
+        _check(_lib.ffbf_get_filter(self._h, which, _ptr(out), words))
         return out
 
     def filter_save(self) -> np.ndarray:

@@ -101,11 +101,27 @@ uint32_t l2_fold_log2(ffbf_ctx *c)
     return g;
 }
 
+void add_fold_job(ffbf_ctx *c, const uint32_t *src, uint64_t src_words, uint32_t *dst, uint32_t dst_words)
+{
+    c->fold_jobs[c->n_fold_jobs++] = ffbf_ctx::FoldJob{src, src_words, dst, dst_words};
+}
+
+// Launches the planned folds, in order, from the finished F.
+int run_fold_jobs(ffbf_ctx *c)
+{
+    for (uint32_t i = 0; i < c->n_fold_jobs; i++) {
+        const ffbf_ctx::FoldJob &j = c->fold_jobs[i];
+        HIP_TRY(ffbf::launch_fold(j.src, j.src_words, j.dst, j.dst_words, c->stream));
+    }
+    return FFBF_OK;
+}
+
 // Picks how the scan pre-tests windows, after F changed or a mode was
 // requested.  Automatic choice: the whole filter in LDS when it fits; else
 // the cheapest of the LDS fold, F probed directly, and (F beyond one XCD's
 // L2) an L2-sized fold, by an expected-cost model over the analytic fills.
-// The folds are computed on the stream; nothing is read back.
+// Plans the folds the mode needs (run_fold_jobs launches them once F is
+// complete); nothing is read back.
 // Split filter: an LDS fold of the cache part (tested by the functions
 // below s) or F probed directly, its cache part first.  The same cost model
 // as below, over the cache part's expected fill.
@@ -152,7 +168,7 @@ int choose_mode_split(ffbf_ctx *c)
     }
     if (fold) {
         RET_IF(c->fold.ensure(words, "fold"));
-        HIP_TRY(ffbf::launch_fold(c->F.p, cache_words, c->fold.p, words, c->stream));
+        add_fold_job(c, c->F.p, cache_words, c->fold.p, words);
         c->mode = ffbf::MODE_FOLD;
         c->pre_words = words;
         c->pre_fill = ff;
@@ -180,10 +196,11 @@ uint32_t auto_probes(const ffbf_ctx *c, double fill)
     return std::min(fill < 0.01 ? 1u : (fill < three_from ? 2u : 3u), k);
 }
 
-int choose_mode(ffbf_ctx *c)
+int plan_mode(ffbf_ctx *c)
 {
     c->generation++;
     c->gfold_words = 0;
+    c->n_fold_jobs = 0;
     if (c->split) return choose_mode_split(c);
     const uint32_t max_words = ffbf::max_fold_words(c->device);
     const uint32_t k = c->params.k;
@@ -202,7 +219,7 @@ int choose_mode(ffbf_ctx *c)
         if (l2 < 10 || l2 > c->params.log2_bits || l2 > 31) return fail(FFBF_EINVAL, "global fold size out of range");
         const uint32_t words = 1u << (l2 - 5);
         RET_IF(c->gfold.ensure(words, "global fold"));
-        HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, words, c->stream));
+        add_fold_job(c, c->F.p, c->f_words, c->gfold.p, words);
         mode = ffbf::MODE_GFOLD;
         c->pre_words = words;
         c->gfold_words = words;
@@ -271,15 +288,15 @@ int choose_mode(ffbf_ctx *c)
                 // reads 4 MiB instead of all of F
                 const uint32_t gw = 1u << (gl2 - 5);
                 RET_IF(c->gfold.ensure(gw, "global fold"));
-                HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, gw, c->stream));
+                add_fold_job(c, c->F.p, c->f_words, c->gfold.p, gw);
                 c->gfold_words = gw;
                 if (gw > words)
-                    HIP_TRY(ffbf::launch_fold(c->gfold.p, gw, c->fold.p, words, c->stream));
+                    add_fold_job(c, c->gfold.p, gw, c->fold.p, words);
                 else
-                    HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
+                    add_fold_job(c, c->F.p, c->f_words, c->fold.p, words);
                 mode = ffbf::MODE_HYBRID;
             } else {
-                HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->fold.p, words, c->stream));
+                add_fold_job(c, c->F.p, c->f_words, c->fold.p, words);
                 mode = ffbf::MODE_FOLD;
             }
             c->pre_words = words;
@@ -290,7 +307,7 @@ int choose_mode(ffbf_ctx *c)
         } else {
             const uint32_t gw = 1u << (gl2 - 5);
             RET_IF(c->gfold.ensure(gw, "global fold"));
-            HIP_TRY(ffbf::launch_fold(c->F.p, c->f_words, c->gfold.p, gw, c->stream));
+            add_fold_job(c, c->F.p, c->f_words, c->gfold.p, gw);
             mode = ffbf::MODE_GFOLD;
             c->pre_words = gw;
             c->gfold_words = gw;
@@ -304,6 +321,13 @@ int choose_mode(ffbf_ctx *c)
     else
         c->probes = ffbf::mode_in_lds(mode) ? auto_probes(c, c->pre_fill) : 1u;
     return FFBF_OK;
+}
+
+// The mode and its folds, from the F now on the stream.
+int choose_mode(ffbf_ctx *c)
+{
+    RET_IF(plan_mode(c));
+    return run_fold_jobs(c);
 }
 
 uint32_t scan_block(ffbf_ctx *c)
@@ -320,6 +344,34 @@ int ensure_cand(ffbf_ctx *c, uint64_t n)
     return FFBF_OK;
 }
 
+// Extra atomics (patterns x k x folds) up to which the atomic build sets the
+// folds' bits itself: 2^17 of them take ~5 us at ~27 G/s, less than a fold
+// launch and the clear it saves.
+constexpr uint64_t kFusedFoldAtomics = 1ull << 17;
+
+// Appends the 8192-word spans of p[0..words) to a clear list.
+void add_spans(std::vector<ffbf::ZeroSpan> &zs, uint32_t *p, uint64_t words)
+{
+    for (uint64_t o = 0; o < words; o += 8192u)
+        zs.push_back(ffbf::ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(8192u, words - o)});
+}
+
+// Puts a clear list on the device unless the copy there (`have`) is the same.
+int upload_spans(ffbf_ctx *c, const std::vector<ffbf::ZeroSpan> &zs, ffbf::DevBuf<ffbf::ZeroSpan> &dst,
+                 std::vector<ffbf::ZeroSpan> &have)
+{
+    if (dst.p && zs.size() == have.size() &&
+        std::equal(zs.begin(), zs.end(), have.begin(),
+                   [](const ffbf::ZeroSpan &a, const ffbf::ZeroSpan &b) { return a.p == b.p && a.words == b.words; }))
+        return FFBF_OK;
+    have.clear();
+    RET_IF(dst.ensure(zs.size(), "clear list"));
+    HIP_TRY(hipMemcpyAsync(dst.p, zs.data(), zs.size() * sizeof(ffbf::ZeroSpan), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    have = zs;
+    return FFBF_OK;
+}
+
 // The spans a scan clears: R, the candidate region counts, the scan's
 // counters (C_CAND, C_SURV, C_MATCH, C_SPILL, C_CVALID) and the survivor
 // index, which the pattern pass fills.
@@ -329,15 +381,11 @@ int build_zero_spans(ffbf_ctx *c)
         c->zero_table_size == c->table_size)
         return FFBF_OK;
     std::vector<ffbf::ZeroSpan> zs;
-    auto zero = [&](uint32_t *p, uint64_t words) {
-        for (uint64_t o = 0; o < words; o += 8192u)
-            zs.push_back(ffbf::ZeroSpan{p + o, (uint32_t)std::min<uint64_t>(8192u, words - o)});
-    };
-    zero(c->R.p, c->f_words);
-    zero(c->cstage.counts.p, c->cstage.n_regions);
-    zero((uint32_t *)(c->counters.p + ffbf::C_CAND), 6u);   // C_CAND, C_SURV, C_MATCH
-    zero((uint32_t *)(c->counters.p + ffbf::C_SPILL), 4u);  // C_SPILL, C_CVALID
-    if (c->table.p) zero(c->table.p, 2ull * c->table_size);
+    add_spans(zs, c->R.p, c->f_words);
+    add_spans(zs, c->cstage.counts.p, c->cstage.n_regions);
+    add_spans(zs, (uint32_t *)(c->counters.p + ffbf::C_CAND), 6u);   // C_CAND, C_SURV, C_MATCH
+    add_spans(zs, (uint32_t *)(c->counters.p + ffbf::C_SPILL), 4u);  // C_SPILL, C_CVALID
+    if (c->table.p) add_spans(zs, c->table.p, 2ull * c->table_size);
     RET_IF(c->d_zero.ensure(zs.size(), "clear list"));
     HIP_TRY(hipMemcpyAsync(c->d_zero.p, zs.data(), zs.size() * sizeof(ffbf::ZeroSpan), hipMemcpyHostToDevice,
                            c->stream));
@@ -775,6 +823,7 @@ void ffbf_destroy(ffbf_ctx *c)
     c->cand.release();
     c->cstage.release();
     c->d_zero.release();
+    c->d_zero_build.release();
     c->matches.release();
     c->sort_keys.release();
     c->sort_temp.release();
@@ -865,6 +914,10 @@ int ffbf_build(ffbf_ctx *c)
     const bool binned = c->build_method == FFBF_BUILD_BINNED ||
                         (c->build_method == FFBF_BUILD_AUTO && slices >= 2 &&
                          c->n_long * c->params.k >= (1ull << 20));
+    // the scan mode and its folds, from the analytic fill (a rejected mode
+    // request leaves the previous build in place)
+    c->f_set_bits = -1;
+    RET_IF(plan_mode(c));
     if (binned) {
         // bits sorted by 2^20-bit slice and set in LDS, each slice written
         // whole (no atomics, no clear), from the prefix hash pairs
@@ -875,14 +928,32 @@ int ffbf_build(ffbf_ctx *c)
         HIP_TRY(ffbf::launch_build_binned(c->off.p, c->pat_hash.p, c->n_pat, c->params.window, c->params.k, c->mask,
                                           c->split, c->smap, c->F.p, c->f_words, c->bin_staging.p, c->bin_offs.p,
                                           c->stream));
+        RET_IF(run_fold_jobs(c));
     } else {
-        HIP_TRY(ffbf::launch_fill32(c->F.p, c->f_words, 0u, c->stream));
+        // few bits: the atomics also set the folds' bits when that adds
+        // fewer atomics than a fold launch costs (two launches fewer: no
+        // separate clear of F, no fold pass)
+        ffbf::BuildFolds fo{};
+        const bool fused = c->n_fold_jobs > 0 && c->n_long * c->params.k * c->n_fold_jobs <= kFusedFoldAtomics;
+        std::vector<ffbf::ZeroSpan> zs;
+        add_spans(zs, c->F.p, c->f_words);
+        if (fused) {
+            // every fold is of F's first src_words words (a fold of a fold of
+            // F, both powers of two, is the fold of F)
+            fo.src_bits = c->fold_jobs[0].src_words * 32u;
+            for (uint32_t i = 0; i < c->n_fold_jobs; i++) {
+                fo.p[i] = c->fold_jobs[i].dst;
+                fo.wmask[i] = c->fold_jobs[i].dst_words - 1u;
+                add_spans(zs, c->fold_jobs[i].dst, c->fold_jobs[i].dst_words);
+            }
+        }
+        RET_IF(upload_spans(c, zs, c->d_zero_build, c->build_zero_host));
+        HIP_TRY(ffbf::launch_zero_spans(c->d_zero_build.p, (uint32_t)zs.size(), c->n_cu, c->stream));
         HIP_TRY(ffbf::launch_build_atomic(c->off.p, c->pat_hash.p, c->n_pat, c->params.window, c->params.k, c->mask,
-                                          c->split, c->smap, c->F.p, c->stream));
+                                          c->split, c->smap, c->F.p, fo, c->stream));
+        if (!fused) RET_IF(run_fold_jobs(c));
     }
     RET_IF(record(c, EV_B1));
-    c->f_set_bits = -1;
-    RET_IF(choose_mode(c));
     c->built = true;
     c->scanned = c->passed = c->verified = false;
     return FFBF_OK;
@@ -1109,12 +1180,24 @@ int ffbf_get_filter(ffbf_ctx *c, int which, uint32_t *out, uint64_t n_words)
 {
     RET_IF(check_ctx(c));
     if (!out) return fail(FFBF_EINVAL, "null output");
-    if (n_words < c->f_words) return fail(FFBF_ECAPACITY, "filter buffer too small");
     const uint32_t *src = nullptr;
-    if (which == FFBF_FILTER_FORWARD) src = c->F.p;
-    else if (which == FFBF_FILTER_REVERSE) src = c->R.p;
-    else return fail(FFBF_EINVAL, "which must be FFBF_FILTER_FORWARD or FFBF_FILTER_REVERSE");
-    HIP_TRY(hipMemcpyAsync(out, src, c->f_words * 4, hipMemcpyDeviceToHost, c->stream));
+    uint64_t words = c->f_words;
+    if (which == FFBF_FILTER_FORWARD) {
+        src = c->F.p;
+    } else if (which == FFBF_FILTER_REVERSE) {
+        src = c->R.p;
+    } else if (which == FFBF_FILTER_FOLD || which == FFBF_FILTER_L2_FOLD) {
+        if (!c->built) return fail(FFBF_ESTATE, "no filter built or loaded");
+        const bool lds = which == FFBF_FILTER_FOLD;
+        if (lds ? (c->mode != ffbf::MODE_FOLD && c->mode != ffbf::MODE_HYBRID) : c->gfold_words == 0)
+            return fail(FFBF_ESTATE, "the scan mode in force has no such fold");
+        src = lds ? c->fold.p : c->gfold.p;
+        words = lds ? c->pre_words : c->gfold_words;
+    } else {
+        return fail(FFBF_EINVAL, "which must be one of FFBF_FILTER_FORWARD .. FFBF_FILTER_L2_FOLD");
+    }
+    if (n_words < words) return fail(FFBF_ECAPACITY, "filter buffer too small");
+    HIP_TRY(hipMemcpyAsync(out, src, words * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return FFBF_OK;
 }

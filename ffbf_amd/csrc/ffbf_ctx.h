@@ -141,6 +141,20 @@ struct ffbf_ctx {
     uint32_t mode = ffbf::MODE_GLOBAL;
     uint32_t pre_words = 0;   // prefilter size in words (F itself, LDS fold or global fold)
     uint32_t gfold_words = 0; // size of the L2-resident fold behind the LDS fold (mode 4)
+    // The OR-folds the chosen mode needs, in order: each folds
+    // src[0..src_words) onto dst[0..dst_words) (powers of two), once F is
+    // complete -- or the atomic build sets their bits beside F's.
+    struct FoldJob {
+        const uint32_t *src;
+        uint64_t src_words;
+        uint32_t *dst;
+        uint32_t dst_words;
+    };
+    FoldJob fold_jobs[2] = {};
+    uint32_t n_fold_jobs = 0;
+    // the atomic build's clear list (F and the folds it sets), as last uploaded
+    std::vector<ffbf::ZeroSpan> build_zero_host;
+    ffbf::DevBuf<ffbf::ZeroSpan> d_zero_build;
     uint32_t probes = 1;      // LDS prefilter bits tested before queueing
     double pre_fill = 0.0;    // expected fraction of prefilter bits set
     uint64_t n_long = 0;      // patterns with length >= W (each sets k bits of F)

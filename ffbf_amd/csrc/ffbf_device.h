@@ -243,11 +243,21 @@ hipError_t launch_prefix_hash(const uint8_t *bytes, const uint64_t *off, uint64_
 // The split method's: 64-bit prefix hashes from a 256-entry (lo, hi) sbox.
 hipError_t launch_prefix_hash_split(const uint8_t *bytes, const uint64_t *off, uint64_t n, const uint32_t *sbox64,
                                     uint32_t W, uint2 *pat_hash, hipStream_t s);
+// Up to two OR-folds of F that the atomic build sets beside F (null p: none).
+// Bit x of F with x < src_bits is bit x mod 32 (wmask + 1) of each: the fold
+// of F's first src_bits bits onto wmask + 1 words (a power of two).
+struct BuildFolds {
+    uint32_t *p[2];
+    uint32_t wmask[2];
+    uint64_t src_bits;
+};
+
 // Atomic build: the k forward bits of every pattern at least W long, set by
-// atomicOr into a cleared F from the prefix hash pairs (placed by m when
-// split).
+// atomicOr into a cleared F (and cleared folds) from the prefix hash pairs
+// (placed by m when split).
 hipError_t launch_build_atomic(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W, uint32_t k,
-                               uint32_t mask, bool split, const SplitMap &m, uint32_t *F, hipStream_t s);
+                               uint32_t mask, bool split, const SplitMap &m, uint32_t *F, const BuildFolds &folds,
+                               hipStream_t s);
 // Atomic-free build (filters of more than one 2^20-bit slice): bit indices of
 // kBinEntries / k patterns per block sorted by slice into `staging`
 // (build_bin_blocks(n, k) * kBinEntries words) with per-block slice offsets in

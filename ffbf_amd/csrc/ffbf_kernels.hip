@@ -110,10 +110,12 @@ __global__ __launch_bounds__(256) void k_build_split(const uint8_t *bytes, const
 // setBitsInFilter, ffbf.cpp:196-203 and 268-283, for the split method): one
 // lane per pattern at least W long sets its k forward bits in a cleared F
 // with atomicOr, from the prefix hash pair computed when the patterns were
-// set; shorter patterns set none (ffbf.cpp:271-275).
+// set; shorter patterns set none (ffbf.cpp:271-275).  The scan's OR-folds of
+// F, when given, get the same bits at x mod their size (the bits k_fold would
+// give them from the finished F).
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_build_atomic(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W,
-                                                      uint32_t k, uint32_t mask, SplitMap m, uint32_t *F)
+                                                      uint32_t k, uint32_t mask, SplitMap m, uint32_t *F, BuildFolds fo)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -122,7 +124,12 @@ __global__ __launch_bounds__(256) void k_build_atomic(const uint64_t *off, const
         for (int j = 0; j < (SPLIT ? kSplitMaxK : 5); j++) {
             if (j >= (int)k) break;
             const uint32_t x = SPLIT ? split_place(j, split_fwd_fn(j, h.x, h.y), m) : (fwd_idx(j, h.x, h.y) & mask);
-            atomicOr(&F[x >> 5], 1u << (x & 31u));
+            const uint32_t bit = 1u << (x & 31u);
+            atomicOr(&F[x >> 5], bit);
+            if ((uint64_t)x < fo.src_bits) {
+                if (fo.p[0]) atomicOr(&fo.p[0][(x >> 5) & fo.wmask[0]], bit);
+                if (fo.p[1]) atomicOr(&fo.p[1][(x >> 5) & fo.wmask[1]], bit);
+            }
         }
     }
 }
@@ -627,17 +634,18 @@ hipError_t launch_prefix_hash_split(const uint8_t *bytes, const uint64_t *off, u
 }
 
 hipError_t launch_build_atomic(const uint64_t *off, const uint2 *pat_hash, uint64_t n, uint32_t W, uint32_t k,
-                               uint32_t mask, bool split, const SplitMap &m, uint32_t *F, hipStream_t s)
+                               uint32_t mask, bool split, const SplitMap &m, uint32_t *F, const BuildFolds &folds,
+                               hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255u) / 256u;
     if (blocks > 8192u) blocks = 8192u;
     if (split)
         hipLaunchKernelGGL(k_build_atomic<true>, dim3((uint32_t)blocks), dim3(256), 0, s, off, pat_hash, n, W, k, mask,
-                           m, F);
+                           m, F, folds);
     else
         hipLaunchKernelGGL(k_build_atomic<false>, dim3((uint32_t)blocks), dim3(256), 0, s, off, pat_hash, n, W, k,
-                           mask, m, F);
+                           mask, m, F, folds);
     return hipGetLastError();
 }
 

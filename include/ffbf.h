@@ -54,6 +54,8 @@ extern "C" {
 
 #define FFBF_FILTER_FORWARD 0 /* F: built from patterns */
 #define FFBF_FILTER_REVERSE 1 /* R: fed forward by the last scan */
+#define FFBF_FILTER_FOLD 2    /* the scan's OR-fold of F held in LDS (modes 1 and 4) */
+#define FFBF_FILTER_L2_FOLD 3 /* the scan's L2-resident OR-fold of F (modes 3 and 4) */
 
 /* Method parameters.  The reference fixes them at compile time: HASH_LEN 19,
  * BLOOMBITS 2^28, four hash functions and its sbox table
@@ -244,7 +246,11 @@ int ffbf_query_line(ffbf_ctx *ctx, const uint8_t *line, uint64_t len, int *hit);
  * receives 0 or 1.  One launch for all lines (one wave per line). */
 int ffbf_query_lines(ffbf_ctx *ctx, const uint8_t *bytes, const uint64_t *offsets, uint64_t n_lines, uint8_t *hits);
 
-/* Filter dump (bvDump, greptest.cu:341-351): n_words must be >= 2^log2_bits/32. */
+/* Filter dump (bvDump, greptest.cu:341-351): n_words must be >= 2^log2_bits/32
+ * for F and R.  The folds (no counterpart in the reference: the scan's
+ * prefilters, for inspection and tests) have 2^fold_log2_bits / 32 and
+ * 2^l2_fold_log2_bits / 32 words (ffbf_get_timings); FFBF_ESTATE when the
+ * scan mode in force has no such fold. */
 int ffbf_get_filter(ffbf_ctx *ctx, int which, uint32_t *out, uint64_t n_words);
 
 /* Candidate offsets of the last scan, ascending (the reference's positions
