@@ -286,6 +286,8 @@ int enqueue_run(ffbf_batch *b, bool timed)
 
 int run_once(ffbf_batch *b)
 {
+    // the run writes its filters' R and survivor indexes
+    for (ffbf_ctx *c : b->filters) c->scan_state_clean = false;
     const bool graph = b->tuning.use_graph != 0u;
     if (graph && !b->exec) {
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

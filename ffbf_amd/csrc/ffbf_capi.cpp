@@ -372,6 +372,25 @@ int upload_spans(ffbf_ctx *c, const std::vector<ffbf::ZeroSpan> &zs, ffbf::DevBu
     return FFBF_OK;
 }
 
+// Appends the scan state a scan starts from zero: R, the candidate region
+// counts, the scan's counters and the survivor index.
+void add_scan_spans(ffbf_ctx *c, std::vector<ffbf::ZeroSpan> &zs)
+{
+    add_spans(zs, c->R.p, c->f_words);
+    if (c->cstage.counts.p) add_spans(zs, c->cstage.counts.p, c->cstage.n_regions);
+    add_spans(zs, (uint32_t *)(c->counters.p + ffbf::C_CAND), 6u);   // C_CAND, C_SURV, C_MATCH
+    add_spans(zs, (uint32_t *)(c->counters.p + ffbf::C_SPILL), 4u);  // C_SPILL, C_CVALID
+    if (c->table.p) add_spans(zs, c->table.p, 2ull * c->table_size);
+}
+
+// The atomic build's clear covered the scan state of the buffers in use now,
+// and nothing has written it since.
+bool scan_state_clean(const ffbf_ctx *c)
+{
+    return c->scan_state_clean && c->clean_key[0] == c->cstage.counts.p && c->clean_size[0] == c->cstage.n_regions &&
+           c->clean_key[1] == c->table.p && c->clean_size[1] == c->table_size;
+}
+
 // The spans a scan clears: R, the candidate region counts, the scan's
 // counters (C_CAND, C_SURV, C_MATCH, C_SPILL, C_CVALID) and the survivor
 // index, which the pattern pass fills.
@@ -381,11 +400,7 @@ int build_zero_spans(ffbf_ctx *c)
         c->zero_table_size == c->table_size)
         return FFBF_OK;
     std::vector<ffbf::ZeroSpan> zs;
-    add_spans(zs, c->R.p, c->f_words);
-    add_spans(zs, c->cstage.counts.p, c->cstage.n_regions);
-    add_spans(zs, (uint32_t *)(c->counters.p + ffbf::C_CAND), 6u);   // C_CAND, C_SURV, C_MATCH
-    add_spans(zs, (uint32_t *)(c->counters.p + ffbf::C_SPILL), 4u);  // C_SPILL, C_CVALID
-    if (c->table.p) add_spans(zs, c->table.p, 2ull * c->table_size);
+    add_scan_spans(c, zs);
     RET_IF(c->d_zero.ensure(zs.size(), "clear list"));
     HIP_TRY(hipMemcpyAsync(c->d_zero.p, zs.data(), zs.size() * sizeof(ffbf::ZeroSpan), hipMemcpyHostToDevice,
                            c->stream));
@@ -403,7 +418,8 @@ int enqueue_scan_begin(ffbf_ctx *c)
 {
     RET_IF(build_zero_spans(c));
     RET_IF(record(c, EV_S0));
-    HIP_TRY(ffbf::launch_zero_spans(c->d_zero.p, c->n_zero, c->n_cu, c->stream));
+    if (!scan_state_clean(c)) HIP_TRY(ffbf::launch_zero_spans(c->d_zero.p, c->n_zero, c->n_cu, c->stream));
+    c->scan_state_clean = false;
     RET_IF(record(c, EV_S1));
     return FFBF_OK;
 }
@@ -467,6 +483,7 @@ void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
 // first window the previous one did not cover and carrying its W-1 byte tail.
 int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end)
 {
+    c->scan_state_clean = false;
     ffbf::ScanArgs a{};
     filter_scan_args(c, a);
     a.text = text;
@@ -494,6 +511,7 @@ int enqueue_scan(ffbf_ctx *c)
 // pass since), so the pass need not clear them again.
 int enqueue_pass(ffbf_ctx *c, bool fresh)
 {
+    c->scan_state_clean = false;
     RET_IF(record(c, EV_P0));
     if (!fresh) {
         HIP_TRY(ffbf::launch_fill32(c->table.p, 2ull * c->table_size, 0u, c->stream));
@@ -522,6 +540,7 @@ int enqueue_pass(ffbf_ctx *c, bool fresh)
 
 int enqueue_verify(ffbf_ctx *c, bool fresh)
 {
+    c->scan_state_clean = false;
     RET_IF(record(c, EV_V0));
     if (!fresh) RET_IF(clear_counters(c, ffbf::C_MATCH, 1));
     ffbf::VerifyArgs a{};
@@ -935,8 +954,11 @@ int ffbf_build(ffbf_ctx *c)
         // separate clear of F, no fold pass)
         ffbf::BuildFolds fo{};
         const bool fused = c->n_fold_jobs > 0 && c->n_long * c->params.k * c->n_fold_jobs <= kFusedFoldAtomics;
+        // one clear for F, the folds the atomics set, and the scan state (the
+        // next scan then skips its own clear)
         std::vector<ffbf::ZeroSpan> zs;
         add_spans(zs, c->F.p, c->f_words);
+        add_scan_spans(c, zs);
         if (fused) {
             // every fold is of F's first src_words words (a fold of a fold of
             // F, both powers of two, is the fold of F)
@@ -949,6 +971,11 @@ int ffbf_build(ffbf_ctx *c)
         }
         RET_IF(upload_spans(c, zs, c->d_zero_build, c->build_zero_host));
         HIP_TRY(ffbf::launch_zero_spans(c->d_zero_build.p, (uint32_t)zs.size(), c->n_cu, c->stream));
+        c->scan_state_clean = true;
+        c->clean_key[0] = c->cstage.counts.p;
+        c->clean_size[0] = c->cstage.n_regions;
+        c->clean_key[1] = c->table.p;
+        c->clean_size[1] = c->table_size;
         HIP_TRY(ffbf::launch_build_atomic(c->off.p, c->pat_hash.p, c->n_pat, c->params.window, c->params.k, c->mask,
                                           c->split, c->smap, c->F.p, fo, c->stream));
         if (!fused) RET_IF(run_fold_jobs(c));

@@ -152,7 +152,15 @@ struct ffbf_ctx {
     };
     FoldJob fold_jobs[2] = {};
     uint32_t n_fold_jobs = 0;
-    // the atomic build's clear list (F and the folds it sets), as last uploaded
+    // Set when the atomic build's clear also covered the scan state (R, the
+    // region counts, the scan's counters, the survivor index) of the buffers
+    // in clean_key / clean_size: the next scan need not clear it again.
+    // Whatever writes that state (a scan, pattern pass, verification, an
+    // allocation-free run, a batch over this filter) clears the flag.
+    bool scan_state_clean = false;
+    const void *clean_key[2] = {nullptr, nullptr};  // region counts and survivor index cleared
+    uint64_t clean_size[2] = {0, 0};
+    // the atomic build's clear list (F, the folds it sets, the scan state), as last uploaded
     std::vector<ffbf::ZeroSpan> build_zero_host;
     ffbf::DevBuf<ffbf::ZeroSpan> d_zero_build;
     uint32_t probes = 1;      // LDS prefilter bits tested before queueing

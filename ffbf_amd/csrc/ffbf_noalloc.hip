@@ -96,6 +96,7 @@ int run_noalloc(ffbf_ctx *c, const Layout &L, const uint8_t *dev_text, uint64_t 
                 uint64_t *n_matches)
 {
     hipStream_t s = c->stream;
+    c->scan_state_clean = false;
     // the array keeps the previous call's contents: clear the counters and
     // the region counts; R and the survivor index are the context's own
     HIP_TRY(launch_fill32((uint32_t *)L.counters, C_COUNT * 2u, 0u, s));

@@ -9,6 +9,7 @@ the build produced, and F must be the oracle's.  Pair method: the LDS fold
 (mode 1, automatic and forced sizes), the L2-resident fold alone (mode 3) and
 both (mode 4, two folds set by one build); split method: the fold of F's
 cache part.  Needs an MI355X."""
+import ctypes
 import sys
 
 import numpy as np
@@ -118,3 +119,71 @@ def test_split_cache_fold(n):
             F = m.filter()
             assert np.array_equal(F, F_ref), method
             assert check_folds(m, F, src_words=(1 << c) // 32) == 1, method
+
+
+def noalloc(m, dev, n, out, cap):
+    nm = ctypes.c_uint64(0)
+    st = fa.lib().ffbf_run_device_noalloc(m._h, dev.ptr, n, out.ptr, cap, ctypes.byref(nm))
+    return st, nm.value
+
+
+def run_and_check(m, dev, text, r):
+    assert m.run_device(dev.ptr, text.size) == r["ids"].size
+    mm = m.matches()
+    assert np.array_equal(mm["pattern_id"], r["ids"]) and np.array_equal(mm["offset"], r["offs"])
+    assert np.array_equal(m.candidates(), r["cand"])
+    assert np.array_equal(m.filter(fa.FILTER_REVERSE), r["R"])
+
+
+def test_scan_after_build_clear():
+    """The atomic build clears the scan state (R, region counts, counters,
+    survivor index) with F, and the scan right after it then skips its own
+    clear.  Every order must still scan from zero: a scan after a build, a
+    scan after a scan, two builds, an allocation-free run or a batch between
+    the build and the scan (both write R and the survivor index)."""
+    W, k, log2 = 19, 4, 24
+    sbox = fa.sbox_from_seed(850)
+    b, off = fa.gen_patterns(fa.ALPHA_ASCII, 851, 10_000, 20, 40)
+    texts = []
+    for s in range(2):
+        t = fa.gen_bytes(fa.ALPHA_ASCII, 852 + s, 3 << 20)
+        fa.plant(860 + s, 300, t, b, off)
+        texts.append(t)
+    o = Oracle(W, k, log2, sbox)
+    refs = [o.run(t, b, off) for t in texts]
+    assert all(r["ids"].size >= 300 for r in refs)
+    devs = [fa.DeviceBuffer.from_host(t) for t in texts]
+    out = fa.DeviceBuffer(4096 * fa.MATCH_DTYPE.itemsize)
+    try:
+        with fa.Matcher(W, k, log2, sbox) as m:
+            m.set_build_method(fa.BUILD_ATOMIC)
+            m.set_patterns(b, off)
+            say("build, scan, scan")
+            m.build()
+            run_and_check(m, devs[0], texts[0], refs[0])
+            run_and_check(m, devs[1], texts[1], refs[1])
+            say("build, build, scan")
+            m.build()
+            m.build()
+            run_and_check(m, devs[0], texts[0], refs[0])
+            say("build, allocation-free run, scan")
+            m.build()
+            assert noalloc(m, devs[1], texts[1].size, out, 4096) == (0, refs[1]["ids"].size)
+            run_and_check(m, devs[0], texts[0], refs[0])
+            say("build, batch, scan")
+            m.build()
+            with fa.Batch([m]) as bt:
+                bt.set_texts(devs[1].ptr, np.array([0, texts[1].size], dtype=np.uint64),
+                             np.zeros(1, dtype=np.uint32))
+                assert bt.run() == refs[1]["ids"].size
+            run_and_check(m, devs[0], texts[0], refs[0])
+            say("build, scan after the build's clear, binned build, scan")
+            m.build()
+            run_and_check(m, devs[1], texts[1], refs[1])
+            m.set_build_method(fa.BUILD_BINNED)
+            m.build()
+            run_and_check(m, devs[0], texts[0], refs[0])
+    finally:
+        out.free()
+        for d in devs:
+            d.free()
