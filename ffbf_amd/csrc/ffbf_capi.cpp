@@ -481,7 +481,9 @@ void filter_scan_args(const ffbf_ctx *c, ffbf::ScanArgs &a)
 // Scans the windows of text[0..n) (device memory), recording their offsets
 // plus base: a text can be scanned piece by piece, each piece starting at the
 // first window the previous one did not cover and carrying its W-1 byte tail.
-int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end)
+// defer_gather: only the compaction's offsets here; the gather goes into the
+// pattern pass's launch (enqueue_pass), which must follow.
+int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end, bool defer_gather)
 {
     c->scan_state_clean = false;
     ffbf::ScanArgs a{};
@@ -493,16 +495,24 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
     a.counters = c->counters.p;
     HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
     if (mark_end) RET_IF(record(c, EV_S2));  // the scan kernel alone ends here
-    HIP_TRY(ffbf::launch_cand_compact(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
-                                      c->cstage.n_regions, c->cstage.region, c->cand.p, c->cand.cap, c->counters.p,
-                                      c->n_cu, c->stream));
+    c->gather_pending = false;
+    if (defer_gather) {
+        HIP_TRY(ffbf::launch_cand_offsets(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
+                                          c->cstage.n_regions, c->cstage.region, c->cand.p, c->cand.cap,
+                                          c->counters.p, c->stream, &c->pending_gather));
+        c->gather_pending = true;
+    } else {
+        HIP_TRY(ffbf::launch_cand_compact(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
+                                          c->cstage.n_regions, c->cstage.region, c->cand.p, c->cand.cap,
+                                          c->counters.p, c->n_cu, c->stream));
+    }
     return FFBF_OK;
 }
 
-int enqueue_scan(ffbf_ctx *c)
+int enqueue_scan(ffbf_ctx *c, bool defer_gather = false)
 {
     RET_IF(enqueue_scan_begin(c));
-    RET_IF(enqueue_scan_range(c, c->text, c->text_n, 0, true));
+    RET_IF(enqueue_scan_range(c, c->text, c->text_n, 0, true, defer_gather));
     RET_IF(record(c, EV_S3));
     return FFBF_OK;
 }
@@ -533,7 +543,12 @@ int enqueue_pass(ffbf_ctx *c, bool fresh)
     a.table_mask = c->table_size - 1u;
     a.method = c->split ? ffbf::METHOD_SPLIT : ffbf::METHOD_PAIR;
     a.split = c->smap;
-    HIP_TRY(ffbf::launch_pattern_pass(a, c->n_cu, c->stream));
+    if (c->gather_pending) {
+        c->gather_pending = false;
+        HIP_TRY(ffbf::launch_pass_gather(a, c->pending_gather, c->n_cu, c->stream));
+    } else {
+        HIP_TRY(ffbf::launch_pattern_pass(a, c->n_cu, c->stream));
+    }
     RET_IF(record(c, EV_P1));
     return FFBF_OK;
 }
@@ -1085,7 +1100,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
 {
     RET_IF(check_ctx(c));
     RET_IF(set_text(c, dev_text, n));
-    RET_IF(enqueue_scan(c));
+    RET_IF(enqueue_scan(c, true));
     RET_IF(enqueue_pass(c, true));
     RET_IF(enqueue_verify(c, true));
     RET_IF(prefetch_matches(c));
@@ -1093,7 +1108,7 @@ int ffbf_run_device(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n, uint64_t *
     if (c->h_counters[ffbf::C_CAND] > c->cand.cap) {
         // rerun the three stages with room for every candidate
         RET_IF(ensure_cand(c, 2 * c->h_counters[ffbf::C_CAND]));
-        RET_IF(enqueue_scan(c));
+        RET_IF(enqueue_scan(c, true));
         RET_IF(enqueue_pass(c, true));
         RET_IF(enqueue_verify(c, true));
         RET_IF(prefetch_matches(c));

@@ -158,6 +158,10 @@ struct ffbf_ctx {
     // Whatever writes that state (a scan, pattern pass, verification, an
     // allocation-free run, a batch over this filter) clears the flag.
     bool scan_state_clean = false;
+    // The fused run path leaves the candidate gather of its scan to the
+    // pattern pass's launch (k_pass_gather): set between the two.
+    bool gather_pending = false;
+    ffbf::GatherArgs pending_gather{};
     const void *clean_key[2] = {nullptr, nullptr};  // region counts and survivor index cleared
     uint64_t clean_size[2] = {0, 0};
     // the atomic build's clear list (F, the folds it sets, the scan state), as last uploaded
@@ -235,7 +239,9 @@ int set_text(ffbf_ctx *c, const uint8_t *dev_text, uint64_t n);
 int record(ffbf_ctx *c, int e);
 int sync_counters(ffbf_ctx *c);
 int enqueue_scan_begin(ffbf_ctx *c);
-int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end);
+// defer_gather: the candidate gather is left to the next enqueue_pass's launch.
+int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t base, bool mark_end,
+                       bool defer_gather = false);
 int do_scan(ffbf_ctx *c);
 // Grows the candidate list (and its staging) to hold at least n entries.
 int ensure_cand(ffbf_ctx *c, uint64_t n);

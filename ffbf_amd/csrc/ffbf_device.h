@@ -289,6 +289,23 @@ hipError_t launch_zero_spans(const ZeroSpan *spans, uint32_t n, int n_cu, hipStr
 hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
                                unsigned long long *counters, int n_cu, hipStream_t s);
+// Where the candidate gather copies from and to (the offsets' results).
+struct GatherArgs {
+    const Cand *raw;
+    const uint64_t *offsets;
+    uint32_t n;  // regions
+    uint32_t region;
+    uint64_t spill_base;
+    Cand *out;
+    uint64_t out_cap;
+};
+// The compaction's first half (offsets, counters, cleared counts); *g
+// receives the gather it leaves to launch_pass_gather.
+hipError_t launch_cand_offsets(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
+                               uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
+                               unsigned long long *counters, hipStream_t s, GatherArgs *g);
+// The pattern pass and the candidate gather g in one launch.
+hipError_t launch_pass_gather(const PassArgs &a, const GatherArgs &g, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);
 hipError_t launch_query_lines(const QueryArgs &a, int n_cu, hipStream_t s);
 

@@ -354,14 +354,15 @@ __device__ __forceinline__ uint32_t pass_wave(const PassArgs &a, uint64_t i0)
     return (uint32_t)__builtin_popcountll(bal);
 }
 
+// The pattern pass as workgroup b of nb (the launch may hold other work).
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
+__device__ __forceinline__ void pattern_pass_blocks(const PassArgs &a, uint32_t b, uint32_t nb)
 {
     __shared__ uint32_t s_n;
     if (threadIdx.x == 0) s_n = 0u;
     __syncthreads();
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t wave0 = ((uint64_t)b * blockDim.x + (threadIdx.x & ~63u));
+    const uint64_t stride = (uint64_t)nb * blockDim.x;
     uint32_t n = 0;
     for (uint64_t i0 = wave0; i0 < a.n; i0 += stride) n += pass_wave<SPLIT>(a, i0);
     // one count per block: atomics on one global address serialise, and a
@@ -369,6 +370,12 @@ __global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
     if (n && lane_id() == 0) atomicAdd(&s_n, n);
     __syncthreads();
     if (threadIdx.x == 0 && s_n) atomicAdd(a.counters + C_SURV, (unsigned long long)s_n);
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_pattern_pass(PassArgs a)
+{
+    pattern_pass_blocks<SPLIT>(a, blockIdx.x, gridDim.x);
 }
 
 // The pattern passes of several filters in one launch: wave_start[f] is the
@@ -841,38 +848,75 @@ __global__ __launch_bounds__(1024) void k_cand_offsets(uint32_t *counts, uint64_
     }
 }
 
-// Appends each region (one wave per region) and then the spill to out.
-__global__ __launch_bounds__(256) void k_cand_gather(const Cand *raw, const uint64_t *offsets, uint32_t n, uint32_t region,
-                                                     uint64_t spill_base, Cand *out, uint64_t out_cap)
+// Appends each region (one wave per region) and then the spill to out, as
+// workgroup b of nb.
+__device__ __forceinline__ void cand_gather_blocks(const GatherArgs &g, uint32_t b, uint32_t nb)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t w = gw; w < n; w += nw) {
-        const uint64_t o = offsets[w], cnt = offsets[w + 1] - o;
+    const uint32_t gw = (b * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nw = (nb * blockDim.x) >> 6;
+    for (uint32_t w = gw; w < g.n; w += nw) {
+        const uint64_t o = g.offsets[w], cnt = g.offsets[w + 1] - o;
         for (uint64_t i = lane; i < cnt; i += 64u)
-            if (o + i < out_cap) out[o + i] = raw[(uint64_t)w * region + i];
+            if (o + i < g.out_cap) g.out[o + i] = g.raw[(uint64_t)w * g.region + i];
     }
-    const uint64_t so = offsets[n], sc = offsets[n + 1];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sc; i += stride)
-        if (so + i < out_cap) out[so + i] = raw[spill_base + i];
+    const uint64_t so = g.offsets[g.n], sc = g.offsets[g.n + 1];
+    const uint64_t stride = (uint64_t)nb * blockDim.x;
+    for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < sc; i += stride)
+        if (so + i < g.out_cap) g.out[so + i] = g.raw[g.spill_base + i];
+}
+
+__global__ __launch_bounds__(256) void k_cand_gather(GatherArgs g) { cand_gather_blocks(g, blockIdx.x, gridDim.x); }
+
+// The pattern pass and the candidate gather in one launch: both read only
+// what the scan and the offsets left (R; the staged candidates), nothing of
+// each other, so workgroups [0, pass_blocks) run the pass and the rest the
+// gather.  One launch fewer on the fused run path.
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_pass_gather(PassArgs a, GatherArgs g, uint32_t pass_blocks)
+{
+    if (blockIdx.x < pass_blocks)
+        pattern_pass_blocks<SPLIT>(a, blockIdx.x, pass_blocks);
+    else
+        cand_gather_blocks(g, blockIdx.x - pass_blocks, gridDim.x - pass_blocks);
 }
 
 }  // namespace
 
-hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
+hipError_t launch_cand_offsets(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
-                               unsigned long long *counters, int n_cu, hipStream_t s)
+                               unsigned long long *counters, hipStream_t s, GatherArgs *g)
 {
     const uint64_t spill_base = (uint64_t)n_regions * region;
     if (raw_cap < spill_base) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cand_offsets, dim3(1), dim3(1024), 0, s, wave_counts, offsets, n_regions, raw_cap - spill_base,
                        out_cap, counters);
-    hipError_t e = hipGetLastError();
+    *g = GatherArgs{raw, offsets, n_regions, region, spill_base, out, out_cap};
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
+                               uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
+                               unsigned long long *counters, int n_cu, hipStream_t s)
+{
+    GatherArgs g;
+    hipError_t e = launch_cand_offsets(raw, raw_cap, wave_counts, offsets, n_regions, region, out, out_cap, counters, s,
+                                       &g);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cand_gather, dim3((uint32_t)(n_cu * 2)), dim3(256), 0, s, raw, offsets, n_regions, region,
-                       spill_base, out, out_cap);
+    hipLaunchKernelGGL(k_cand_gather, dim3((uint32_t)(n_cu * 2)), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_gather(const PassArgs &a, const GatherArgs &g, int n_cu, hipStream_t s)
+{
+    uint64_t pass_blocks = (a.n + 255u) / 256u;
+    const uint64_t cap = (uint64_t)n_cu * 8u;
+    if (pass_blocks > cap) pass_blocks = cap;
+    const uint32_t grid = (uint32_t)pass_blocks + (uint32_t)(n_cu * 2);
+    if (a.method == METHOD_SPLIT)
+        hipLaunchKernelGGL(k_pass_gather<true>, dim3(grid), dim3(256), 0, s, a, g, (uint32_t)pass_blocks);
+    else
+        hipLaunchKernelGGL(k_pass_gather<false>, dim3(grid), dim3(256), 0, s, a, g, (uint32_t)pass_blocks);
     return hipGetLastError();
 }
 

@@ -98,9 +98,10 @@ typedef struct ffbf_match {
  * measured with HIP events on the context's stream. */
 typedef struct ffbf_timings {
     float build_ms;        /* filter build (clear F, hash prefixes, set bits) */
-    float scan_ms;         /* clears + scan with feed-forward + candidate compaction */
+    float scan_ms;         /* clears + scan with feed-forward + candidate compaction (ffbf_run_device:
+                              its offsets only; the gather shares the pattern pass's launch) */
     float scan_kernel_ms;  /* the scan kernel alone */
-    float pattern_pass_ms; /* pattern pass (+ survivor index) */
+    float pattern_pass_ms; /* pattern pass (+ survivor index; + the candidate gather in ffbf_run_device) */
     float verify_ms;       /* exact verification */
     uint32_t scan_mode;      /* 0 F probed in HBM/L2, 1 LDS fold, 2 whole F in LDS, 3 L2-resident fold,
                                 4 LDS fold then L2-resident fold */
