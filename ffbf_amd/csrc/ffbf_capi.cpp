@@ -493,18 +493,19 @@ int enqueue_scan_range(ffbf_ctx *c, const uint8_t *text, uint64_t n, uint64_t ba
     a.off_base = base;
     c->cstage.fill(a, 0u, c->cstage.n_regions);
     a.counters = c->counters.p;
-    HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream));
+    uint32_t waves = 0;
+    HIP_TRY(ffbf::launch_scan(a, c->mode, scan_block(c), c->n_cu, c->stream, &waves));
     if (mark_end) RET_IF(record(c, EV_S2));  // the scan kernel alone ends here
     c->gather_pending = false;
     if (defer_gather) {
         HIP_TRY(ffbf::launch_cand_offsets(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
                                           c->cstage.n_regions, c->cstage.region, c->cand.p, c->cand.cap,
-                                          c->counters.p, c->stream, &c->pending_gather));
+                                          c->counters.p, c->stream, &c->pending_gather, waves));
         c->gather_pending = true;
     } else {
         HIP_TRY(ffbf::launch_cand_compact(c->cstage.raw.p, c->cstage.raw.cap, c->cstage.counts.p, c->cstage.offsets.p,
                                           c->cstage.n_regions, c->cstage.region, c->cand.p, c->cand.cap,
-                                          c->counters.p, c->n_cu, c->stream));
+                                          c->counters.p, c->n_cu, c->stream, waves));
     }
     return FFBF_OK;
 }

@@ -271,7 +271,10 @@ hipError_t launch_build_binned(const uint64_t *off, const uint2 *pat_hash, uint6
                                uint32_t mask, bool split, SplitMap m, uint32_t *F, uint64_t f_words, uint32_t *staging,
                                uint32_t *offs, hipStream_t s);
 hipError_t launch_fold(const uint32_t *F, uint64_t f_words, uint32_t *fold, uint32_t fold_words, hipStream_t s);
-hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s);
+// waves (when given): the waves launched, whose region slots a.wave0 .. +waves
+// the launch may fill
+hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s,
+                       uint32_t *waves = nullptr);
 // Random-access microbenchmark (kind: FFBF_PROBE_* of ffbf.h).
 hipError_t launch_gather(uint32_t *arr, uint32_t log2_words, uint32_t steps, uint32_t *out, uint32_t blocks,
                          uint32_t kind, hipStream_t s);
@@ -286,9 +289,11 @@ hipError_t launch_zero_spans(const ZeroSpan *spans, uint32_t n, int n_cu, hipStr
 // Appends the regions and spill of the raw candidate buffer to the list
 // out[counters[C_CAND]..], updates C_CAND (beyond out_cap when anything was
 // lost) and zeroes the region counts and C_SPILL for the next launch.
+// n_used (at most n_regions): only region slots [0, n_used) can hold
+// entries (the launch's waves); the spill follows all n_regions slots.
 hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
-                               unsigned long long *counters, int n_cu, hipStream_t s);
+                               unsigned long long *counters, int n_cu, hipStream_t s, uint32_t n_used = UINT32_MAX);
 // Where the candidate gather copies from and to (the offsets' results).
 struct GatherArgs {
     const Cand *raw;
@@ -303,7 +308,7 @@ struct GatherArgs {
 // receives the gather it leaves to launch_pass_gather.
 hipError_t launch_cand_offsets(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
-                               unsigned long long *counters, hipStream_t s, GatherArgs *g);
+                               unsigned long long *counters, hipStream_t s, GatherArgs *g, uint32_t n_used = UINT32_MAX);
 // The pattern pass and the candidate gather g in one launch.
 hipError_t launch_pass_gather(const PassArgs &a, const GatherArgs &g, int n_cu, hipStream_t s);
 hipError_t launch_verify(const VerifyArgs &a, int n_cu, hipStream_t s);

@@ -885,23 +885,26 @@ __global__ __launch_bounds__(256) void k_pass_gather(PassArgs a, GatherArgs g, u
 
 hipError_t launch_cand_offsets(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
-                               unsigned long long *counters, hipStream_t s, GatherArgs *g)
+                               unsigned long long *counters, hipStream_t s, GatherArgs *g, uint32_t n_used)
 {
     const uint64_t spill_base = (uint64_t)n_regions * region;
     if (raw_cap < spill_base) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cand_offsets, dim3(1), dim3(1024), 0, s, wave_counts, offsets, n_regions, raw_cap - spill_base,
-                       out_cap, counters);
-    *g = GatherArgs{raw, offsets, n_regions, region, spill_base, out, out_cap};
+    // offsets[n] and offsets[n + 1] describe the spill, so offsets needs n + 2
+    // entries for any n <= n_regions
+    const uint32_t n = n_used < n_regions ? n_used : n_regions;
+    hipLaunchKernelGGL(k_cand_offsets, dim3(1), dim3(1024), 0, s, wave_counts, offsets, n, raw_cap - spill_base, out_cap,
+                       counters);
+    *g = GatherArgs{raw, offsets, n, region, spill_base, out, out_cap};
     return hipGetLastError();
 }
 
 hipError_t launch_cand_compact(const Cand *raw, uint64_t raw_cap, uint32_t *wave_counts, uint64_t *offsets,
                                uint32_t n_regions, uint32_t region, Cand *out, uint64_t out_cap,
-                               unsigned long long *counters, int n_cu, hipStream_t s)
+                               unsigned long long *counters, int n_cu, hipStream_t s, uint32_t n_used)
 {
     GatherArgs g;
     hipError_t e = launch_cand_offsets(raw, raw_cap, wave_counts, offsets, n_regions, region, out, out_cap, counters, s,
-                                       &g);
+                                       &g, n_used);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cand_gather, dim3((uint32_t)(n_cu * 2)), dim3(256), 0, s, g);
     return hipGetLastError();

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a)
 }
 
 template <int WT, bool LDSPRE, int BLOCK, int P, bool SPLIT = false, int UPL = 1>
-hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
+hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s, uint32_t *waves)
 {
     const size_t dyn = LDSPRE ? (size_t)a.pre_words * 4u * (a.r_lds ? 2u : 1u) : 0;
     if (dyn > 0) {
@@ -163,6 +163,7 @@ hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
     const uint64_t need = (iters + (BLOCK / 64) - 1) / (BLOCK / 64);
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
+    if (waves) *waves = (uint32_t)grid * (uint32_t)(BLOCK / 64);
     hipLaunchKernelGGL((k_scan<WT, LDSPRE, BLOCK, P, SPLIT, UPL>), dim3((uint32_t)grid), dim3(BLOCK), dyn, s, a);
     return hipGetLastError();
 }
@@ -170,36 +171,36 @@ hipError_t launch_scan_t(const ScanArgs &a, int n_cu, hipStream_t s)
 // Split method: the cache-part fold in LDS (1024-thread blocks, P <= 3 bits
 // up front) or F's first bit read directly (256-thread blocks).
 template <int WT>
-hipError_t launch_scan_split_w(const ScanArgs &a, uint32_t mode, int n_cu, hipStream_t s)
+hipError_t launch_scan_split_w(const ScanArgs &a, uint32_t mode, int n_cu, hipStream_t s, uint32_t *waves)
 {
     if (mode == MODE_FOLD) {
         const uint32_t P = a.P < 1u ? 1u : (a.P > 3u ? 3u : a.P);
-        if (P == 1) return launch_scan_t<WT, true, 1024, 1, true>(a, n_cu, s);
-        if (P == 2) return launch_scan_t<WT, true, 1024, 2, true>(a, n_cu, s);
-        return launch_scan_t<WT, true, 1024, 3, true>(a, n_cu, s);
+        if (P == 1) return launch_scan_t<WT, true, 1024, 1, true>(a, n_cu, s, waves);
+        if (P == 2) return launch_scan_t<WT, true, 1024, 2, true>(a, n_cu, s, waves);
+        return launch_scan_t<WT, true, 1024, 3, true>(a, n_cu, s, waves);
     }
-    return launch_scan_t<WT, false, 256, 1, true>(a, n_cu, s);
+    return launch_scan_t<WT, false, 256, 1, true>(a, n_cu, s, waves);
 }
 
 // LDS prefilter: 1024-thread blocks (512 where a smaller LDS forces it), P
 // bits tested up front; global prefilter: 256-thread blocks, one bit.
 template <int WT>
-hipError_t launch_scan_w(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s)
+hipError_t launch_scan_w(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s, uint32_t *waves)
 {
     const bool lds = mode_in_lds(mode);
     if (lds) {
         const uint32_t P = a.P < 1u ? 1u : (a.P > 3u ? 3u : a.P);
         if (block == 512) {
-            if (P == 1) return launch_scan_t<WT, true, 512, 1>(a, n_cu, s);
-            if (P == 2) return launch_scan_t<WT, true, 512, 2>(a, n_cu, s);
-            return launch_scan_t<WT, true, 512, 3>(a, n_cu, s);
+            if (P == 1) return launch_scan_t<WT, true, 512, 1>(a, n_cu, s, waves);
+            if (P == 2) return launch_scan_t<WT, true, 512, 2>(a, n_cu, s, waves);
+            return launch_scan_t<WT, true, 512, 3>(a, n_cu, s, waves);
         }
         constexpr int U = WT > 0 ? 2 : 1;
-        if (P == 1) return launch_scan_t<WT, true, 1024, 1, false, U>(a, n_cu, s);
-        if (P == 2) return launch_scan_t<WT, true, 1024, 2, false, U>(a, n_cu, s);
-        return launch_scan_t<WT, true, 1024, 3, false, U>(a, n_cu, s);
+        if (P == 1) return launch_scan_t<WT, true, 1024, 1, false, U>(a, n_cu, s, waves);
+        if (P == 2) return launch_scan_t<WT, true, 1024, 2, false, U>(a, n_cu, s, waves);
+        return launch_scan_t<WT, true, 1024, 3, false, U>(a, n_cu, s, waves);
     }
-    return launch_scan_t<WT, false, 256, 1>(a, n_cu, s);
+    return launch_scan_t<WT, false, 256, 1>(a, n_cu, s, waves);
 }
 
 // Random dword gathers into a 2^log2_words-word array: 16 independent loads
@@ -267,17 +268,17 @@ uint32_t max_fold_words(int device, bool split)
     return words;
 }
 
-hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s)
+hipError_t launch_scan(const ScanArgs &a, uint32_t mode, uint32_t block, int n_cu, hipStream_t s, uint32_t *waves)
 {
     if (a.method == METHOD_SPLIT) {
         if (mode != MODE_FOLD && mode != MODE_GLOBAL) return hipErrorInvalidValue;
-        return a.W == 19 ? launch_scan_split_w<19>(a, mode, n_cu, s) : launch_scan_split_w<0>(a, mode, n_cu, s);
+        return a.W == 19 ? launch_scan_split_w<19>(a, mode, n_cu, s, waves) : launch_scan_split_w<0>(a, mode, n_cu, s, waves);
     }
     switch (a.W) {
-    case 19: return launch_scan_w<19>(a, mode, block, n_cu, s);
-    case 24: return launch_scan_w<24>(a, mode, block, n_cu, s);
-    case 32: return launch_scan_w<32>(a, mode, block, n_cu, s);
-    default: return launch_scan_w<0>(a, mode, block, n_cu, s);
+    case 19: return launch_scan_w<19>(a, mode, block, n_cu, s, waves);
+    case 24: return launch_scan_w<24>(a, mode, block, n_cu, s, waves);
+    case 32: return launch_scan_w<32>(a, mode, block, n_cu, s, waves);
+    default: return launch_scan_w<0>(a, mode, block, n_cu, s, waves);
     }
 }
 
